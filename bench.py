@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the split-step Crank-Nicolson dipolar GPE solver on one MI355X.
+
+Workload (default): plain imaginary-time FP64 solve at 256^3, dx = dy = dz =
+0.1 (box [-12.8, 12.8)^3), dt = 5e-4, gamma/nu/lambda = 0.5/1/1.5, G0 = 40,
+GDD0 = 30 (OPTION = 2: g = 80, gd = 60), spherical cut-off R = 12.8, Gaussian
+initial wave function.  A step is one full split step (dipolar convolution,
+calcnu, x/y/z Crank-Nicolson sweeps, Simpson norm and renormalisation) over
+the whole grid with psi resident in HBM.
+
+Prints one JSON line: grid-pt-steps/s = Nx*Ny*Nz*K / t over K timed steps,
+the roofline of the dominant kernel (algorithmic bytes / mean launch time,
+measured with HIP events on the solver's stream) and the CPU baseline (the
+FP64 oracle on the host cores, bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "dbec-gp-mpi-cuda_amd")
+sys.path.insert(0, os.path.join(PKG, "python"))
+
+METRIC = "grid-pt-steps/s at 256³ (FP64 vs mixed) + solves/s, p50/p99 at |Δμ/μ|≤1e-8"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    # name: (n, h, dt, cutoff, mode, prec)
+    "imag256_fp64": ((256, 256, 256), 0.1, 0.0005, 12.8, "imag", "fp64"),
+    "imag256_mixed": ((256, 256, 256), 0.1, 0.0005, 12.8, "imag", "mixed"),
+    "real256x384_fp64": ((256, 256, 384), 0.1, 0.0002, 12.8, "real", "fp64"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def b_alg_per_point(mode, prec, nz):
+    """Algorithmic HBM bytes per grid point per split step of the reference
+    operator sequence: each operator one streaming pass, one half-spectrum
+    read+write per FFT axis with the x forward * potdd * inverse fused, Thomas
+    intermediates on chip.  imag: 15e + 8ch + eh; real: 21e + 8ch + eh, with
+    e the real element size, c the complex one, h = (Nz/2+1)/Nz."""
+    e = 8 if prec == "fp64" else 4
+    c = 2 * e
+    h = (nz // 2 + 1) / nz
+    base = 15 * e if mode == "imag" else 21 * e
+    return base + 8 * c * h + e * h
+
+
+def cpu_baseline(n, h, dt, cutoff, mode, steps, threads):
+    """The FP64 oracle (OpenMP) on the host, a bounded sample of the same workload."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import dbecpy
+
+    lib = dbecpy.load(os.path.join(REPO, "oracle", "liboracle.so"))
+    p = dbecpy.make_params(n[0], n[1], n[2], h, h, h, dt, 0.5, 1.0, 1.5, cutoff)
+    m = dbecpy.IMAG if mode == "imag" else dbecpy.REAL
+    with dbecpy.Solver(lib, p, m) as s:
+        s.set_psi(dbecpy.gaussian_psi0(p))
+        s.calcnorm(rescale=(m == dbecpy.IMAG))
+        s.set_coupling(80.0, 60.0)
+        s.step(1)  # first touch of every array
+        t0 = time.perf_counter()
+        s.step(steps)
+        t = time.perf_counter() - t0
+    return n[0] * n[1] * n[2] * steps / t, t
+
+
+def traffic_for(workload, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (profiles/traffic_<workload>.json), or None."""
+    path = os.path.join(REPO, "profiles", f"traffic_{workload}.json")
+    try:
+        d = json.load(open(path))
+        k = d["kernels"][kernel]
+        return k["fetch_bytes"] + k["write_bytes"]
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="imag256_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    rank = int(os.environ.get("RANK", "0"))
+    if args.gpus != 1 or world != 1:
+        if rank == 0:
+            print("bench.py: this build runs on one GPU; a single solve does not shard across GPUs here "
+                  "(see DESIGN.md, 'How the path would shard'). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import dbecpy
+
+    n, h, dt, cutoff, mode, prec = WORKLOADS[args.workload]
+    lib = dbecpy.load(dbecpy.HIP_LIB)
+    p = dbecpy.make_params(n[0], n[1], n[2], h, h, h, dt, 0.5, 1.0, 1.5, cutoff, device=0)
+    m = dbecpy.IMAG if mode == "imag" else dbecpy.REAL
+    pr = dbecpy.FP64 if prec == "fp64" else dbecpy.MIXED
+    npts = n[0] * n[1] * n[2]
+    log(f"workload {args.workload}: grid {n}, {mode} time, {prec}")
+    s = dbecpy.Solver(lib, p, m, pr)
+    s.set_psi(dbecpy.gaussian_psi0(p))
+    s.calcnorm(rescale=(m == dbecpy.IMAG))
+    s.set_coupling(80.0, 60.0)
+    s.step(args.warmup, want_norm=False)
+    s.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    s.step(args.steps, want_norm=False)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: {t:.3f} s")
+    value = npts * args.steps / t
+    ms_step = 1e3 * t / args.steps
+    norm = s.calcnorm()
+    mu, en = s.calcmuen()
+
+    stats = s.profile(args.profile_reps)
+    log("per-kernel: " + ", ".join(f"{k['name']} {k['avg_ms'] * 1e3:.1f}us" for k in stats))
+    dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
+    achieved = dom["alg_bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    step_kernel_ms = sum(k["avg_ms"] * k["launches"] for k in stats)
+    s.close()
+
+    balg = b_alg_per_point(mode, prec, n[2])
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{mode}-time dipolar GPE split step, {n[0]}x{n[1]}x{n[2]}, {prec}",
+            "grid": list(n), "dx": h, "dt": dt, "trap": [0.5, 1.0, 1.5], "G0": 40.0, "GDD0": 30.0,
+            "cutoff": cutoff, "option": 2, "psi0": "Gaussian",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom["name"],
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_for(args.workload, dom["name"]),
+            "alg_bytes_per_launch": dom["alg_bytes"],
+            "avg_launch_ms": dom["avg_ms"],
+        },
+        "step_roofline": {
+            "b_alg_bytes_per_point": balg,
+            "achieved_gbs": balg * npts / (ms_step * 1e-3) / 1e9,
+            "frac": balg * npts / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernel_sum_ms": step_kernel_ms,
+        },
+        "kernels": {k["name"]: {"avg_us": k["avg_ms"] * 1e3, "gbs": k["alg_bytes"] / (k["avg_ms"] * 1e-3) / 1e9,
+                                "launches_per_step": k["launches"]} for k in stats},
+        "observables": {"norm": norm, "mu_over_par": mu / 2.0, "E_over_par": en / 2.0},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle, {args.cpu_steps} steps at {n}, {threads} threads")
+        v, tc = cpu_baseline(n, h, dt, cutoff, mode, args.cpu_steps, threads)
+        line["cpu_baseline"] = {
+            "value": v, "unit": "grid-pt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} FP64 split steps of the same {n[0]}x{n[1]}x{n[2]} {mode}-time workload "
+                      f"({tc:.1f} s), OpenMP oracle",
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,290 @@
+// fft_kernels.hpp - the passes of the dipolar convolution
+//   phi = IDFT3( potdd * DFT3( |psi|^2 ) )      (unnormalised)
+// over the half spectrum spec[kz][x][y], kz = 0..Nz/2.
+//
+//   zfwd : reads psi rows [x][y][:], forms |s psi|^2 (s = pending
+//          renormalisation), transforms two real rows at once as the real and
+//          imaginary parts of one complex line, splits the result into the two
+//          half spectra and writes spec[kz][x][y-tile] (coalesced over y).
+//   ypass: contiguous y lines of spec, forward or inverse.
+//   xconv: x lines of spec (stride Ny, coalesced over y tiles): forward FFT,
+//          multiplication by potdd(|kx|, |ky|, kz), inverse FFT, one pass.
+//   zinv : rebuilds two Hermitian full spectra per complex line, inverse
+//          transform, real and imaginary parts are the two real rows of phi.
+//
+// Included by fft_inst.hip with DBEC_FFT_N set; every kernel is a template on
+// the length N, the lines per workgroup L and the real type R of spec.
+#pragma once
+
+#include "internal.hpp"
+
+namespace dbec {
+
+template <int N>
+struct FftTile {
+    static constexpr int T = FftPlan<N>::T;
+    // lines per workgroup: at least 8, and at least one wavefront of threads
+    static constexpr int L0 = (64 + T - 1) / T;
+    static constexpr int L = L0 <= 8 ? 8 : (L0 <= 16 ? 16 : (L0 <= 32 ? 32 : 64));
+    static constexpr int WG = L * T;
+};
+
+template <typename R>
+__device__ __forceinline__ void stage_twiddles(cpx<R> *tw_s, const cpx<R> *tw_g, int n, int tid, int nthreads) {
+    for (int i = tid; i < n; i += nthreads) tw_s[i] = tw_g[i];
+}
+
+template <typename P>
+struct PsiTraits;
+template <>
+struct PsiTraits<double> {
+    static __device__ __forceinline__ double mod2(double v, double s) {
+        double t = v * s;
+        return t * t;
+    }
+};
+template <>
+struct PsiTraits<float> {
+    static __device__ __forceinline__ double mod2(float v, double s) {
+        double t = (double)v * s;
+        return t * t;
+    }
+};
+template <>
+struct PsiTraits<cpx<double>> {
+    static __device__ __forceinline__ double mod2(cpx<double> v, double) {
+        double a = hypot(v.x, v.y);
+        return a * a;
+    }
+};
+template <>
+struct PsiTraits<cpx<float>> {
+    static __device__ __forceinline__ double mod2(cpx<float> v, double) {
+        double a = hypot((double)v.x, (double)v.y);
+        return a * a;
+    }
+};
+
+// ---------------------------------------------------------------------------
+template <int N, typename R, typename P>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__restrict__ spec,
+       int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    const int ytiles = (Ny + 2 * L - 1) / (2 * L);
+    const int x = blockIdx.x / ytiles;
+    const int y0 = (blockIdx.x % ytiles) * 2 * L;
+    const int nrows = min(2 * L, Ny - y0);
+    const double s = scale ? *scale : 1.0;
+    stage_twiddles(tw, twg, N, tid, WG);
+    const P *src = psi + ((long)x * Ny + y0) * N;
+    for (int i = tid; i < 2 * L * N; i += WG) {
+        const int z = i % N, row = i / N;
+        R v = R(0);
+        if (row < nrows) v = (R)PsiTraits<P>::mod2(src[(long)row * N + z], s);
+        R *slot = reinterpret_cast<R *>(&buf[z * LS + (row >> 1)]);
+        slot[row & 1] = v;
+    }
+    __syncthreads();
+    FftPlan<N>::template run<-1>(buf, LS, tid % L, tid / L, tw);
+    // split Z = A + iB into the spectra of the two real rows:
+    //   A[k] = (Z[k] + conj Z[N-k]) / 2,  B[k] = (Z[k] - conj Z[N-k]) / (2i)
+    const int nkz = N / 2 + 1;
+    for (int i = tid; i < nkz * 2 * L; i += WG) {
+        const int row = i % (2 * L), k = i / (2 * L);
+        if (row < nrows) {
+            const int p = row >> 1;
+            const cpx<R> z = buf[k * LS + p];
+            const cpx<R> w = buf[((N - k) % N) * LS + p];
+            cpx<R> o;
+            if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
+            else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
+            spec[((long)k * Nx + x) * Ny + y0 + row] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int N, int DIR, typename R>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    stage_twiddles(tw, twg, N, tid, WG);
+    const long line0 = (long)blockIdx.x * L;
+    const int nl = (int)min((long)L, nlines - line0);
+    cpx<R> *g = spec + line0 * N;
+    for (int i = tid; i < nl * N; i += WG) {
+        const int e = i % N, l = i / N;
+        buf[e * LS + l] = g[i];
+    }
+    __syncthreads();
+    FftPlan<N>::template run<DIR>(buf, LS, tid % L, tid / L, tw);
+    for (int i = tid; i < nl * N; i += WG) {
+        const int e = i % N, l = i / N;
+        g[i] = buf[e * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    const int ytiles = (Ny + L - 1) / L;
+    const int kz = blockIdx.x / ytiles;
+    const int y0 = (blockIdx.x % ytiles) * L;
+    const int nl = min(L, Ny - y0);
+    stage_twiddles(tw, twg, N, tid, WG);
+    cpx<R> *g = spec + (long)kz * N * Ny + y0;
+    for (int i = tid; i < N * L; i += WG) {
+        const int l = i % L, e = i / L;
+        if (l < nl) buf[e * LS + l] = g[(long)e * Ny + l];
+    }
+    __syncthreads();
+    const int l = tid % L, t = tid / L;
+    FftPlan<N>::template run<-1>(buf, LS, l, t, tw);
+    const int Ny2 = Ny / 2, Nyh = Ny2 + 1, Nxh = N / 2 + 1;
+    const R *pk = potddq + (long)kz * Nxh * Nyh;
+    for (int i = tid; i < N * L; i += WG) {
+        const int ll = i % L, e = i / L;
+        const int y = y0 + ll;
+        if (ll < nl) {
+            const int kx = e <= N / 2 ? e : N - e;
+            const int ky = y <= Ny2 ? y : Ny - y;
+            const R w = pk[kx * Nyh + ky];
+            cpx<R> v = buf[e * LS + ll];
+            buf[e * LS + ll] = {v.x * w, v.y * w};
+        }
+    }
+    __syncthreads();
+    FftPlan<N>::template run<+1>(buf, LS, l, t, tw);
+    for (int i = tid; i < N * L; i += WG) {
+        const int ll = i % L, e = i / L;
+        if (ll < nl) g[(long)e * Ny + ll] = buf[e * LS + ll];
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    const int ytiles = (Ny + 2 * L - 1) / (2 * L);
+    const int x = blockIdx.x / ytiles;
+    const int y0 = (blockIdx.x % ytiles) * 2 * L;
+    const int nrows = min(2 * L, Ny - y0);
+    stage_twiddles(tw, twg, N, tid, WG);
+    // Z[k] = A[k] + i B[k] (k <= N/2) and Z[N-k] = conj A[k] + i conj B[k];
+    // the imaginary parts of the self-conjugate bins k = 0, N/2 are dropped.
+    const int nkz = N / 2 + 1;
+    for (int i = tid; i < nkz * L; i += WG) {
+        const int p = i % L, k = i / L;
+        const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (2 * p < nrows) a = spec[base];
+        if (2 * p + 1 < nrows) b = spec[base + 1];
+        if (k == 0 || k == N / 2) {
+            a.y = R(0);
+            b.y = R(0);
+        }
+        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
+        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+    }
+    __syncthreads();
+    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    R *dst = phi + ((long)x * Ny + y0) * N;
+    for (int i = tid; i < 2 * L * N; i += WG) {
+        const int z = i % N, row = i / N;
+        if (row < nrows) {
+            const cpx<R> c = buf[z * LS + (row >> 1)];
+            dst[(long)row * N + z] = (row & 1) ? c.y : c.x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers for one length.
+template <int N>
+struct FftLaunch {
+    using Tl = FftTile<N>;
+    static hipError_t zfwd(const FftArgs &a) {
+        const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
+        dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
+        if (!a.fp32) {
+            auto *spec = static_cast<cpx<double> *>(a.spec);
+            auto *tw = static_cast<const cpx<double> *>(a.tw);
+            if (a.cpsi)
+                k_zfwd<N, double, cpx<double>><<<grid, block, 0, a.stream>>>(
+                    static_cast<const cpx<double> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+            else
+                k_zfwd<N, double, double><<<grid, block, 0, a.stream>>>(
+                    static_cast<const double *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+        } else {
+            auto *spec = static_cast<cpx<float> *>(a.spec);
+            auto *tw = static_cast<const cpx<float> *>(a.tw);
+            if (a.cpsi)
+                k_zfwd<N, float, cpx<float>><<<grid, block, 0, a.stream>>>(
+                    static_cast<const cpx<float> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+            else
+                k_zfwd<N, float, float><<<grid, block, 0, a.stream>>>(
+                    static_cast<const float *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+        }
+        return hipGetLastError();
+    }
+    static hipError_t ypass(int dir, const FftArgs &a) {
+        const long nlines = a.Nkz * a.Nx;
+        dim3 grid((unsigned)((nlines + Tl::L - 1) / Tl::L)), block(Tl::WG);
+        if (!a.fp32) {
+            auto *spec = static_cast<cpx<double> *>(a.spec);
+            auto *tw = static_cast<const cpx<double> *>(a.tw);
+            if (dir < 0) k_ypass<N, -1, double><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            else k_ypass<N, +1, double><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+        } else {
+            auto *spec = static_cast<cpx<float> *>(a.spec);
+            auto *tw = static_cast<const cpx<float> *>(a.tw);
+            if (dir < 0) k_ypass<N, -1, float><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            else k_ypass<N, +1, float><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+        }
+        return hipGetLastError();
+    }
+    static hipError_t xconv(const FftArgs &a) {
+        const int ytiles = (int)((a.Ny + Tl::L - 1) / Tl::L);
+        dim3 grid((unsigned)(a.Nkz * ytiles)), block(Tl::WG);
+        if (!a.fp32)
+            k_xconv<N, double><<<grid, block, 0, a.stream>>>(static_cast<cpx<double> *>(a.spec),
+                                                             static_cast<const double *>(a.potddq), (int)a.Ny,
+                                                             static_cast<const cpx<double> *>(a.tw));
+        else
+            k_xconv<N, float><<<grid, block, 0, a.stream>>>(static_cast<cpx<float> *>(a.spec),
+                                                            static_cast<const float *>(a.potddq), (int)a.Ny,
+                                                            static_cast<const cpx<float> *>(a.tw));
+        return hipGetLastError();
+    }
+    static hipError_t zinv(const FftArgs &a) {
+        const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
+        dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
+        if (!a.fp32)
+            k_zinv<N, double><<<grid, block, 0, a.stream>>>(static_cast<const cpx<double> *>(a.spec),
+                                                            static_cast<double *>(a.phi), (int)a.Nx, (int)a.Ny,
+                                                            static_cast<const cpx<double> *>(a.tw));
+        else
+            k_zinv<N, float><<<grid, block, 0, a.stream>>>(static_cast<const cpx<float> *>(a.spec),
+                                                           static_cast<float *>(a.phi), (int)a.Nx, (int)a.Ny,
+                                                           static_cast<const cpx<float> *>(a.tw));
+        return hipGetLastError();
+    }
+};
+
+}  // namespace dbec

@@ -1,0 +1,37 @@
+// ops.hpp - launch interfaces of the non-FFT kernels (ops.hip).
+#pragma once
+
+#include <algorithm>
+#include <type_traits>
+
+#include "internal.hpp"
+
+namespace dbec {
+
+struct TrapConst {
+    double half_par, g2, n2, l2;  // 0.5*par, gamma^2, nu^2, lambda^2
+};
+struct NuConst {
+    double dt, g, gd, nd;  // nd = Nx*Ny*Nz
+};
+struct MuenConst {
+    double dx, dy, dz, kin, g, gd, nd;  // kin = 3 - par
+};
+template <typename V>
+struct SweepConst {
+    V A0r, A, mA;  // diagonal of the explicit half, off-diagonal, and its negative
+};
+
+hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny, long Nz);
+hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale);
+hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale);
+hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *rows);
+hipError_t launch_rows_mod2(hipStream_t s, const dbec_ctx *c, double *rows0, double *rowsz);
+hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren);
+// mode 0: norm and 1/norm; 1: norm and rms moments; 2: mu and E
+hipError_t launch_finalize(hipStream_t s, const dbec_ctx *c, int mode, const double *A, const double *B);
+hipError_t launch_reduce3(hipStream_t s, const dbec_ctx *c, int axis, double *out);
+hipError_t launch_reduce2(hipStream_t s, const double *in, long n0, long n1, int axis, const double *w, double *out);
+hipError_t launch_mod2(hipStream_t s, const dbec_ctx *c, double *out);
+
+}  // namespace dbec

@@ -1,0 +1,147 @@
+"""Parity of the HIP library with the FP64 oracle, through the C ABI.
+
+Per operator on small grids (every transform length family), multi-step
+trajectories (imaginary and real time), and the reference's example chain via
+the product programs.  FP64 tolerance: 1e-10 relative in mu, E, rms and
+max|d psi| (the bar of the plain path); single operators are checked at 1e-12.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dbecpy
+from conftest import params, rel, smooth_field
+from golden_cli import check_chain, run_chain
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-12
+TOL = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+GRIDS = [(32, 48, 64), (64, 64, 64), (48, 80, 96), (128, 96, 80), (160, 192, 16)]
+
+
+@pytest.mark.parametrize("mode", [dbecpy.IMAG, dbecpy.REAL], ids=["imag", "real"])
+@pytest.mark.parametrize("n", GRIDS, ids=lambda n: "x".join(map(str, n)))
+def test_operator_parity(hip_lib, oracle_lib, mode, n):
+    say(f"operator parity mode={mode} grid={n}")
+    p = params(n=n, h=(0.25, 0.2, 0.3), dt=0.004, cutoff=5.0)
+    f = smooth_field(p, seed=7, complex_=(mode == dbecpy.REAL))
+    with dbecpy.Solver(hip_lib, p, mode) as g, dbecpy.Solver(oracle_lib, p, mode) as o:
+        for s in (g, o):
+            s.set_psi(f)
+            s.set_coupling(80.0, 60.0)
+        for s in (g, o):
+            s.calcpsidd2()
+        assert rel(g.get_psidd2(), o.get_psidd2()) < TOL_OP, "calcpsidd2"
+        for name in ("calcnu", "calclux", "calcluy", "calcluz"):
+            for s in (g, o):
+                getattr(s, name)()
+            assert rel(g.get_psi(), o.get_psi()) < TOL_OP, name
+        ng, no = g.calcnorm(rescale=(mode == dbecpy.IMAG)), o.calcnorm(rescale=(mode == dbecpy.IMAG))
+        assert ng == pytest.approx(no, rel=TOL_OP)
+        assert rel(g.get_psi(), o.get_psi()) < TOL_OP, "calcnorm rescale"
+        mg, mo = g.calcmuen(), o.calcmuen()
+        assert mg == pytest.approx(mo, rel=TOL_OP)
+        assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL_OP)
+        assert g.psi_center() == pytest.approx(o.psi_center(), rel=TOL_OP, abs=1e-300)
+        for kind in range(10):
+            for o3 in ((1, 1, 1), (2, 3, 2)):
+                a, b = g.density(kind, *o3), o.density(kind, *o3)
+                assert a.shape == b.shape
+                assert rel(a, b) < TOL_OP, (kind, o3)
+
+
+@pytest.mark.parametrize("mode", [dbecpy.IMAG, dbecpy.REAL], ids=["imag", "real"])
+def test_tables_match(hip_lib, oracle_lib, mode):
+    say(f"tables mode={mode}")
+    p = params(n=(32, 48, 64))
+    with dbecpy.Solver(hip_lib, p, mode) as g, dbecpy.Solver(oracle_lib, p, mode) as o:
+        assert np.array_equal(g.table(dbecpy.TABLE_POT), o.table(dbecpy.TABLE_POT))
+        assert np.array_equal(g.table(dbecpy.TABLE_POTDD), o.table(dbecpy.TABLE_POTDD))
+        for ax in range(3):
+            for t in (dbecpy.TABLE_CALPHA, dbecpy.TABLE_CGAMMA):
+                assert rel(g.table(t, ax), o.table(t, ax)) < 1e-15
+
+
+def test_imag_trajectory_cfg0(hip_lib, oracle_lib):
+    """64^3, dx=0.25, dt=0.002, 2000 steps, gamma/nu/lambda = 0.5/1/1.5, G0=40,
+    GDD0=30, R=8, Gaussian psi0: mu, E, rms every 500 steps and max|d psi|."""
+    say("imag trajectory 64^3 x 2000")
+    p = dbecpy.make_params(64, 64, 64, 0.25, 0.25, 0.25, 0.002, 0.5, 1.0, 1.5, 8.0)
+    psi0 = dbecpy.gaussian_psi0(p)
+    with dbecpy.Solver(hip_lib, p) as g, dbecpy.Solver(oracle_lib, p) as o:
+        for s in (g, o):
+            s.set_psi(psi0)
+            s.calcnorm(rescale=True)
+            s.set_coupling(2 * 40.0, 2 * 30.0)
+        for chunk in range(4):
+            ng, no = g.step(500), o.step(500)
+            assert ng == pytest.approx(no, rel=TOL)
+            assert g.calcmuen() == pytest.approx(o.calcmuen(), rel=TOL)
+            assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
+            d = rel(g.get_psi(), o.get_psi())
+            say(f"  after {500 * (chunk + 1)} steps: max|dpsi|/max|psi| = {d:.3e}")
+            assert d < TOL
+
+
+def test_imag_ramp_matches(hip_lib, oracle_lib):
+    """NSTP ramp: g += g_stp before every step, accumulated."""
+    say("imag ramp 48x64x32")
+    p = params(n=(48, 64, 32), h=(0.3, 0.25, 0.3), dt=0.003, cutoff=6.0)
+    with dbecpy.Solver(hip_lib, p) as g, dbecpy.Solver(oracle_lib, p) as o:
+        for s in (g, o):
+            s.set_psi(dbecpy.gaussian_psi0(p))
+            s.calcnorm(rescale=True)
+            s.set_coupling(0.0, 0.0)
+            s.step(37, 2 * 40.0 / 37, 2 * 30.0 / 37)
+        assert g.get_coupling() == o.get_coupling()
+        assert rel(g.get_psi(), o.get_psi()) < TOL
+
+
+def test_real_trajectory(hip_lib, oracle_lib):
+    say("real trajectory 48x64x80 x 300")
+    p = params(n=(48, 64, 80), h=(0.3, 0.25, 0.2), dt=0.001, cutoff=6.0)
+    f = dbecpy.gaussian_psi0(p).astype(np.complex128) * np.exp(1j * 0.2 * dbecpy.grid(p)[0][:, None, None])
+    with dbecpy.Solver(hip_lib, p, dbecpy.REAL) as g, dbecpy.Solver(oracle_lib, p, dbecpy.REAL) as o:
+        for s in (g, o):
+            s.set_psi(f)
+            s.set_coupling(2 * 40.0, 2 * 45.0)
+        for _ in range(3):
+            g.step(100, want_norm=False)
+            o.step(100, want_norm=False)
+            assert rel(g.get_psi(), o.get_psi()) < TOL
+            assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
+            assert g.calcnorm() == pytest.approx(o.calcnorm(), rel=TOL)
+
+
+def test_imag_256_short(hip_lib, oracle_lib):
+    """The 256^3 workload (dx = 0.1, dt = 5e-4, R = 12.8), 8 steps."""
+    say("imag 256^3 x 8")
+    p = dbecpy.make_params(256, 256, 256, 0.1, 0.1, 0.1, 0.0005, 0.5, 1.0, 1.5, 12.8)
+    psi0 = dbecpy.gaussian_psi0(p)
+    with dbecpy.Solver(hip_lib, p) as g, dbecpy.Solver(oracle_lib, p) as o:
+        for s in (g, o):
+            s.set_psi(psi0)
+            s.calcnorm(rescale=True)
+            s.set_coupling(80.0, 60.0)
+        assert g.step(8) == pytest.approx(o.step(8), rel=TOL)
+        assert rel(g.get_psi(), o.get_psi()) < TOL
+        assert g.calcmuen() == pytest.approx(o.calcmuen(), rel=TOL)
+
+
+def test_reference_chain_product_programs(tmp_path):
+    """The product's imag3d -> real3d chain on the reference's example inputs
+    against the reference's shipped outputs."""
+    say("reference chain via product programs")
+    work = run_chain(dbecpy.BIN_DIR, str(tmp_path), timeout=300)
+    rep = check_chain(work, den_tol=1e-11)
+    say(f"  last-digit flips: { {k: v for k, v in rep.items() if k.endswith('.txt')} }")
