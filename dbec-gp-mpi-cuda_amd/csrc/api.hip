@@ -50,6 +50,22 @@ hipError_t launch_zinv(long n, const FftArgs &a) {
     default: return hipErrorInvalidValue;
     }
 }
+hipError_t launch_zinv_nu(long n, const FftArgs &a) {
+    switch (n) {
+#define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a, false);
+        DBEC_FOR_EACH_FFT_LEN(DBEC_X)
+#undef DBEC_X
+    default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_face_nu(long n, const FftArgs &a) {
+    switch (n) {
+#define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a, true);
+        DBEC_FOR_EACH_FFT_LEN(DBEC_X)
+#undef DBEC_X
+    default: return hipErrorInvalidValue;
+    }
+}
 
 // Per-kernel event timing used by dbec_profile_step.
 struct KernelTimer {
@@ -123,6 +139,9 @@ FftArgs fft_args(const dbec_ctx *c, int axis, const double *scale) {
     a.phi = c->b.phi;
     a.potddq = c->b.potddq;
     a.tw = a.fp32 ? c->b.twf[axis] : c->b.tw[axis];
+    a.sq = c->b.sq;
+    a.tc = trap_const(c);
+    a.nc = nu_const(c);
     return a;
 }
 
@@ -145,10 +164,30 @@ int run_calcpsidd2(dbec_ctx *c, const double *scale) {
     return DBEC_OK;
 }
 
-int run_sweep(dbec_ctx *c, int axis, double *rows) {
+// The composite step's convolution: the forward passes, then the inverse z
+// pass fused with calcnu (interior rows) and the face rows.
+int run_conv_nu(dbec_ctx *c, const double *scale) {
+    const double ce = (double)(2 * real_elem(c));
+    const double spec_b = (double)c->Nkz * c->Nx * c->Ny * ce;
+    HIPCHK(c, timed(c, "zfwd", (double)c->N * psi_elem(c) + spec_b, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, scale)); }), "zfwd");
+    HIPCHK(c, timed(c, "ypass_fwd", 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, nullptr)); }), "ypass");
+    const double pq = (double)c->Nkz * (c->Nx / 2 + 1) * (c->Ny / 2 + 1) * real_elem(c);
+    HIPCHK(c, timed(c, "xconv", 2 * spec_b + pq, [&] { return launch_xconv(c->Nx, fft_args(c, 0, nullptr)); }), "xconv");
+    HIPCHK(c, timed(c, "ypass_inv", 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, nullptr)); }), "ypass");
+    const double inner = (double)(c->Nx - 1) * (c->Ny - 1) / ((double)c->Nx * c->Ny);
+    HIPCHK(c, timed(c, "zinv_nu", inner * (spec_b + 2.0 * c->N * psi_elem(c)),
+                    [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, scale)); }), "zinv_nu");
+    const double nf = (double)(c->Nx + c->Ny - 1);
+    HIPCHK(c, timed(c, "face_nu", nf * c->Nkz * 2 * ce + nf * c->Nz * 2.0 * psi_elem(c),
+                    [&] { return launch_face_nu(c->Nz, fft_args(c, 2, scale)); }), "face_nu");
+    c->phi_valid = false;
+    return DBEC_OK;
+}
+
+int run_sweep(dbec_ctx *c, int axis, double *colpart) {
     static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
-    double bytes = 2.0 * (double)c->N * psi_elem(c) + (rows ? (double)c->Nx * c->Ny * 8.0 : 0.0);
-    HIPCHK(c, timed(c, names[axis], bytes, [&] { return launch_sweep(c->stream, c, axis, rows); }), "sweep");
+    double bytes = 2.0 * (double)c->N * psi_elem(c) + (colpart ? (double)c->Nx * sweep_z_tiles(c) * 8.0 : 0.0);
+    HIPCHK(c, timed(c, names[axis], bytes, [&] { return launch_sweep(c->stream, c, axis, colpart); }), "sweep");
     return DBEC_OK;
 }
 
@@ -628,15 +667,14 @@ int dbec_step(dbec_ctx *c, long nsteps, double g_stp, double gd_stp, double *nor
             c->gd += gd_stp;
         }
         const double *scale = c->scale_pending ? c->b.scal + SC_SCALE : nullptr;
-        CHK(run_calcpsidd2(c, scale));
-        CHK(run_calcnu(c, scale));
+        CHK(run_conv_nu(c, scale));
         c->scale_pending = false;
         CHK(run_sweep(c, 0, nullptr));
         CHK(run_sweep(c, 1, nullptr));
         CHK(run_sweep(c, 2, imag ? c->b.rows : nullptr));
         if (imag) {
-            HIPCHK(c, timed(c, "norm", (double)c->Nx * c->Ny * 8.0,
-                            [&] { return launch_finalize(c->stream, c, 0, c->b.rows, nullptr); }), "norm");
+            HIPCHK(c, timed(c, "norm", (double)c->Nx * sweep_z_tiles(c) * 8.0,
+                            [&] { return launch_norm_from_cols(c->stream, c, c->b.rows); }), "norm");
             c->scale_pending = true;
         }
     }

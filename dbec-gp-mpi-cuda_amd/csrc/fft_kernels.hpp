@@ -214,6 +214,115 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// zinv fused with calcnu (calcnu_kernel, imag3d.cu:1759-1783; real3d.cu
+// 1856-1883) for the interior rows x < Nx-1, y < Ny-1.  Within such a row the
+// face copy of calcpsidd2_kernel2b (imag3d.cu:1618-1624) takes phi[.][.][Nz-1]
+// from phi[.][.][0] of the same row, so the whole row is available on chip.
+template <int N, typename R, typename P>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
+          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    const int ytiles = (Ny + 2 * L - 1) / (2 * L);
+    const int x = blockIdx.x / ytiles;  // grid covers x < Nx - 1
+    const int y0 = (blockIdx.x % ytiles) * 2 * L;
+    const int nrows = min(2 * L, Ny - y0);
+    stage_twiddles(tw, twg, N, tid, WG);
+    const int nkz = N / 2 + 1;
+    for (int i = tid; i < nkz * L; i += WG) {
+        const int p = i % L, k = i / L;
+        const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (2 * p < nrows) a = spec[base];
+        if (2 * p + 1 < nrows) b = spec[base + 1];
+        if (k == 0 || k == N / 2) {
+            a.y = R(0);
+            b.y = R(0);
+        }
+        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
+        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+    }
+    __syncthreads();
+    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    const double s = scale ? *scale : 1.0;
+    const double x2 = sq[x];
+    P *dst = psi + ((long)x * Ny + y0) * N;
+    for (int i = tid; i < 2 * L * N; i += WG) {
+        const int z = i % N, row = i / N;
+        const int y = y0 + row;
+        if (row < nrows && y < Ny - 1) {
+            const cpx<R> c = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
+            const double phi = (double)((row & 1) ? c.y : c.x);
+            const double pot = trap(tc, x2, sq[Nx + y], sq[Nx + Ny + z]);
+            const long n = (long)row * N + z;
+            dst[n] = nu_apply(dst[n], s, phi, pot, nc);
+        }
+    }
+}
+
+// calcnu on the face rows, whose phi comes from other rows after the
+// periodic face copies (calcpsidd2_kernel2a/2b, imag3d.cu:1591-1625):
+//   row (Nx-1, y):          z < Nz-1 and y < Ny-1 from row (0, y), else own
+//   row (x, Ny-1), x<Nx-1:  z < Nz-1 from row (x, 0), z = Nz-1 own.
+// Each face row is transformed together with its source row as the real and
+// imaginary parts of one complex line.  Face f < Ny is (Nx-1, f); face
+// f >= Ny is (f - Ny, Ny-1).
+template <int N, typename R, typename P>
+__global__ void __launch_bounds__(FftTile<N>::WG)
+k_face_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
+          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> tw[N];
+    const int tid = threadIdx.x;
+    const int nface = Ny + Nx - 1;
+    const int f0 = blockIdx.x * L;
+    const int nl = min(L, nface - f0);
+    stage_twiddles(tw, twg, N, tid, WG);
+    const int nkz = N / 2 + 1;
+    for (int i = tid; i < nkz * L; i += WG) {
+        const int p = i % L, k = i / L;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (p < nl) {
+            const int f = f0 + p;
+            int xo, yo, xs, ys;
+            if (f < Ny) { xo = Nx - 1; yo = f; xs = 0; ys = f; }
+            else { xo = f - Ny; yo = Ny - 1; xs = f - Ny; ys = 0; }
+            a = spec[((long)k * Nx + xo) * Ny + yo];
+            b = spec[((long)k * Nx + xs) * Ny + ys];
+        }
+        if (k == 0 || k == N / 2) {
+            a.y = R(0);
+            b.y = R(0);
+        }
+        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
+        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+    }
+    __syncthreads();
+    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    const double s = scale ? *scale : 1.0;
+    for (int i = tid; i < L * N; i += WG) {
+        const int z = i % N, p = i / N;
+        if (p < nl) {
+            const int f = f0 + p;
+            int xo, yo;
+            bool from_src;
+            if (f < Ny) { xo = Nx - 1; yo = f; from_src = (f < Ny - 1) && (z < N - 1); }
+            else { xo = f - Ny; yo = Ny - 1; from_src = (z < N - 1); }
+            const cpx<R> c = buf[z * LS + p];
+            const double phi = (double)(from_src ? c.y : c.x);
+            const double pot = trap(tc, sq[xo], sq[Nx + yo], sq[Nx + Ny + z]);
+            const long n = ((long)xo * Ny + yo) * N + z;
+            psi[n] = nu_apply(psi[n], s, phi, pot, nc);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers for one length.
 template <int N>
@@ -284,6 +393,26 @@ struct FftLaunch {
                                                            static_cast<float *>(a.phi), (int)a.Nx, (int)a.Ny,
                                                            static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
+    }
+    template <typename P, typename R>
+    static hipError_t fused_t(const FftArgs &a, bool faces) {
+        const auto *spec = static_cast<const cpx<R> *>(a.spec);
+        auto *psi = static_cast<P *>(a.psi);
+        const auto *tw = static_cast<const cpx<R> *>(a.tw);
+        if (!faces) {
+            const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
+            dim3 grid((unsigned)((a.Nx - 1) * ytiles)), block(Tl::WG);
+            k_zinv_nu<N, R, P><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx, (int)a.Ny, tw);
+        } else {
+            const long nface = a.Ny + a.Nx - 1;
+            dim3 grid((unsigned)((nface + Tl::L - 1) / Tl::L)), block(Tl::WG);
+            k_face_nu<N, R, P><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx, (int)a.Ny, tw);
+        }
+        return hipGetLastError();
+    }
+    static hipError_t fused(const FftArgs &a, bool faces) {
+        if (!a.fp32) return a.cpsi ? fused_t<cpx<double>, double>(a, faces) : fused_t<double, double>(a, faces);
+        return a.cpsi ? fused_t<cpx<float>, float>(a, faces) : fused_t<float, float>(a, faces);
     }
 };
 

@@ -9,6 +9,7 @@
 
 #include "dbec.h"
 #include "fft_device.hpp"
+#include "nu_device.hpp"
 
 namespace dbec {
 
@@ -81,11 +82,19 @@ struct FftArgs {
     void *phi;
     const void *potddq;
     const void *tw;       // twiddles of the transformed axis (FP64 or FP32)
+    const double *sq;     // x^2 | y^2 | z^2 (fused calcnu)
+    TrapConst tc;
+    NuConst nc;
 };
 // Each returns hipSuccess or an error; `n` is the transformed length.
 hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C along z)
 hipError_t launch_ypass(long n, int dir, const FftArgs &a);
 hipError_t launch_xconv(long n, const FftArgs &a);   // x FFT, * potdd, inverse x FFT
 hipError_t launch_zinv(long n, const FftArgs &a);    // spec -> phi (C2R along z)
+// C2R along z fused with calcnu for the rows x < Nx-1, y < Ny-1 (phi never
+// leaves the chip), and the same for the face rows x = Nx-1 or y = Ny-1 with
+// the reference's periodic face copies of phi.
+hipError_t launch_zinv_nu(long n, const FftArgs &a);
+hipError_t launch_face_nu(long n, const FftArgs &a);
 
 }  // namespace dbec

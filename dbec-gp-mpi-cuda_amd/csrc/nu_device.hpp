@@ -1,0 +1,65 @@
+// nu_device.hpp - the nonlinear + trap exponential (calcnu) as device
+// functions, shared by the standalone calcnu kernel and the fused
+// inverse-FFT kernels.
+//
+// Reference: calcnu_kernel, src/imag3d-mpicuda/imag3d-mpicuda.cu:1759-1783
+// (imaginary time) and src/real3d-mpicuda/real3d-mpicuda.cu:1856-1883 (real
+// time); trap potential initpot imag3d-mpicuda.cu:872-889.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+
+namespace dbec {
+
+struct TrapConst {
+    double half_par, g2, n2, l2;  // 0.5*par, gamma^2, nu^2, lambda^2
+};
+struct NuConst {
+    double dt, g, gd, nd;  // nd = Nx*Ny*Nz
+};
+
+// 0.5*par*(g^2 x^2 + n^2 y^2 + l^2 z^2) in the reference's evaluation order
+// with no contraction, i.e. the number the reference's pot table holds.
+__device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2, double z2) {
+#pragma clang fp contract(off)
+    return tc.half_par * (tc.g2 * x2 + tc.n2 * y2 + tc.l2 * z2);
+}
+
+// imaginary time: psi * exp(-dt (pot + g psi^2 + gd phi / N))
+__device__ __forceinline__ double nu_imag(double psitmp, double phi, double pot, const NuConst &nc) {
+    const double psi2lin = psitmp * psitmp * nc.g;
+    const double psidd2lin = (phi / nc.nd) * nc.gd;
+    const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
+    return psitmp * exp(-pottmp);
+}
+
+// real time: psi * e^{-i dt (pot + g |psi|^2 + gd phi / N)}
+__device__ __forceinline__ cpx<double> nu_real(cpx<double> v, double phi, double pot, const NuConst &nc) {
+    const double a = hypot(v.x, v.y);
+    const double psi2lin = a * a * nc.g;
+    const double psidd2lin = (phi / nc.nd) * nc.gd;
+    const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
+    double sn, cs;
+    sincos(-pottmp, &sn, &cs);
+    return {v.x * cs - v.y * sn, v.x * sn + v.y * cs};
+}
+
+// Generic over the storage of psi: double / float (imaginary time, with the
+// pending renormalisation s) and cpx<double> / cpx<float> (real time).
+__device__ __forceinline__ double nu_apply(double v, double s, double phi, double pot, const NuConst &nc) {
+    return nu_imag(v * s, phi, pot, nc);
+}
+__device__ __forceinline__ float nu_apply(float v, double s, double phi, double pot, const NuConst &nc) {
+    return (float)nu_imag((double)v * s, phi, pot, nc);
+}
+__device__ __forceinline__ cpx<double> nu_apply(cpx<double> v, double, double phi, double pot, const NuConst &nc) {
+    return nu_real(v, phi, pot, nc);
+}
+__device__ __forceinline__ cpx<float> nu_apply(cpx<float> v, double, double phi, double pot, const NuConst &nc) {
+    cpx<double> r = nu_real(cpx<double>{(double)v.x, (double)v.y}, phi, pot, nc);
+    return {(float)r.x, (float)r.y};
+}
+
+}  // namespace dbec

@@ -42,90 +42,64 @@ hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny,
 }
 
 // ---------------------------------------------------------------------------
-// Trap potential recomputed from the grid instead of read from a table
-// (initpot, imag3d.cu:872-889): 0.5*par*(g^2 x^2 + n^2 y^2 + l^2 z^2) with the
-// reference's evaluation order and no contraction, so the value is the one the
-// table would hold.
-__device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2, double z2) {
-#pragma clang fp contract(off)
-    return tc.half_par * (tc.g2 * x2 + tc.n2 * y2 + tc.l2 * z2);
-}
-
-// calcnu, imaginary time (calcnu_kernel imag3d.cu:1759-1783):
-//   psi <- psi * exp(-dt (pot + g psi^2 + gd phi / (Nx Ny Nz)))
-// with psi read as psi * scale when a renormalisation is pending.
+// calcnu (calcnu_kernel imag3d.cu:1759-1783; real3d.cu:1856-1883), with psi
+// read as psi * scale when a renormalisation is pending (imaginary time).  The
+// trap is recomputed from the grid (nu_device.hpp).
 template <typename P, typename R>
 __global__ void __launch_bounds__(256)
-k_calcnu_imag(P *__restrict__ psi, const R *__restrict__ phi, const double *__restrict__ scale,
-              const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int Nz) {
+k_calcnu(P *__restrict__ psi, const R *__restrict__ phi, const double *__restrict__ scale,
+         const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int Nz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Nz) return;
     const long row = blockIdx.y;  // x * Ny + y
     const int i = (int)(row / Ny), j = (int)(row % Ny);
     const long n = row * Nz + k;
     const double s = scale ? *scale : 1.0;
-    const double psitmp = (double)psi[n] * s;
     const double pot = trap(tc, sq[i], sq[Nx + j], sq[Nx + Ny + k]);
-    const double psi2lin = psitmp * psitmp * nc.g;
-    const double psidd2lin = ((double)phi[n] / nc.nd) * nc.gd;
-    const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
-    psi[n] = (P)(psitmp * exp(-pottmp));
+    psi[n] = nu_apply(psi[n], s, (double)phi[n], pot, nc);
 }
 
-// calcnu, real time (real3d.cu:1856-1883): psi <- psi * e^{-i dt V}.
-template <typename P, typename R>
-__global__ void __launch_bounds__(256)
-k_calcnu_real(P *__restrict__ psi, const R *__restrict__ phi, const double *__restrict__ sq, TrapConst tc,
-              NuConst nc, int Nx, int Ny, int Nz) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Nz) return;
-    const long row = blockIdx.y;
-    const int i = (int)(row / Ny), j = (int)(row % Ny);
-    const long n = row * Nz + k;
-    const P v = psi[n];
-    const double re = (double)v.x, im = (double)v.y;
-    const double a = hypot(re, im);
-    const double pot = trap(tc, sq[i], sq[Nx + j], sq[Nx + Ny + k]);
-    const double psi2lin = a * a * nc.g;
-    const double psidd2lin = ((double)phi[n] / nc.nd) * nc.gd;
-    const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
-    double sn, cs;
-    sincos(-pottmp, &sn, &cs);
-    psi[n] = P{(decltype(v.x))(re * cs - im * sn), (decltype(v.x))(re * sn + im * cs)};
-}
-
-hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale) {
-    const auto &b = c->b;
+TrapConst trap_const(const dbec_ctx *c) {
     TrapConst tc;
     tc.half_par = 0.5 * c->p.par;
     tc.g2 = c->p.vgamma * c->p.vgamma;
     tc.n2 = c->p.vnu * c->p.vnu;
     tc.l2 = c->p.vlambda * c->p.vlambda;
+    return tc;
+}
+
+NuConst nu_const(const dbec_ctx *c) {
     NuConst nc;
     nc.dt = c->p.dt;
     nc.g = c->g;
     nc.gd = c->gd;
     nc.nd = (double)(c->Nx * c->Ny * c->Nz);
+    return nc;
+}
+
+hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale) {
+    const auto &b = c->b;
+    const TrapConst tc = trap_const(c);
+    const NuConst nc = nu_const(c);
     dim3 block(256), grid((unsigned)((c->Nz + 255) / 256), (unsigned)(c->Nx * c->Ny));
+    const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
     const bool f32 = c->prec == DBEC_MIXED;
     if (c->mode == DBEC_IMAG) {
         if (!f32)
-            k_calcnu_imag<double, double><<<grid, block, 0, s>>>(static_cast<double *>(b.psi),
-                                                                 static_cast<const double *>(b.phi), scale, b.sq,
-                                                                 tc, nc, (int)c->Nx, (int)c->Ny, (int)c->Nz);
+            k_calcnu<double, double><<<grid, block, 0, s>>>(static_cast<double *>(b.psi), static_cast<const double *>(b.phi),
+                                                            scale, b.sq, tc, nc, Nx, Ny, Nz);
         else
-            k_calcnu_imag<float, float><<<grid, block, 0, s>>>(static_cast<float *>(b.psi),
-                                                               static_cast<const float *>(b.phi), scale, b.sq, tc,
-                                                               nc, (int)c->Nx, (int)c->Ny, (int)c->Nz);
+            k_calcnu<float, float><<<grid, block, 0, s>>>(static_cast<float *>(b.psi), static_cast<const float *>(b.phi),
+                                                          scale, b.sq, tc, nc, Nx, Ny, Nz);
     } else {
         if (!f32)
-            k_calcnu_real<cpx<double>, double><<<grid, block, 0, s>>>(
-                static_cast<cpx<double> *>(b.psi), static_cast<const double *>(b.phi), b.sq, tc, nc, (int)c->Nx,
-                (int)c->Ny, (int)c->Nz);
+            k_calcnu<cpx<double>, double><<<grid, block, 0, s>>>(static_cast<cpx<double> *>(b.psi),
+                                                                 static_cast<const double *>(b.phi), nullptr, b.sq, tc,
+                                                                 nc, Nx, Ny, Nz);
         else
-            k_calcnu_real<cpx<float>, float><<<grid, block, 0, s>>>(
-                static_cast<cpx<float> *>(b.psi), static_cast<const float *>(b.phi), b.sq, tc, nc, (int)c->Nx,
-                (int)c->Ny, (int)c->Nz);
+            k_calcnu<cpx<float>, float><<<grid, block, 0, s>>>(static_cast<cpx<float> *>(b.psi),
+                                                               static_cast<const float *>(b.phi), nullptr, b.sq, tc, nc,
+                                                               Nx, Ny, Nz);
     }
     return hipGetLastError();
 }
@@ -163,9 +137,11 @@ hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale) {
 // backward recurrence overwrites slot i-1 with cbeta[i-1] right after reading
 // psi[i-1], and the forward substitution overwrites slot i with the new
 // psi[i] after reading cbeta[i], so one LDS array per line suffices and no
-// intermediate touches HBM.  With rows != nullptr (z sweep, imaginary time)
-// the lane also accumulates the Simpson z integral of the new psi^2 for the
-// norm (calcnorm, imag3d.cu:1194-1209).
+// intermediate touches HBM.  The LDS reads of each recurrence do not depend
+// on it, so they are issued in batches of U ahead of the dependent
+// arithmetic.  With COLS (z sweep, imaginary time) the lane also accumulates
+// the Simpson z integral of the new psi^2 and the wavefront sums the tile's
+// rows with the y weights: colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
 __device__ __forceinline__ double m_mul(double a, double b) { return a * b; }
 __device__ __forceinline__ double m_add(double a, double b) { return a + b; }
 __device__ __forceinline__ double m_sub(double a, double b) { return a - b; }
@@ -202,11 +178,79 @@ template <typename S> __device__ __forceinline__ S from_a(cpx<double> v);
 template <> __device__ __forceinline__ cpx<double> from_a<cpx<double>>(cpx<double> v) { return v; }
 template <> __device__ __forceinline__ cpx<float> from_a<cpx<float>>(cpx<double> v) { return {(float)v.x, (float)v.y}; }
 
-template <typename S, int AXIS, int L>
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One line in LDS (stride LS): the reference recurrence, reads batched by U.
+template <typename A, int LS, bool COLS>
+__device__ __forceinline__ double thomas_line(A *__restrict__ f, int n, const A *__restrict__ calpha,
+                                              const A *__restrict__ cgamma, const SweepConst<A> &k,
+                                              const double *__restrict__ wz) {
+    constexpr int U = 8;
+    // backward: cbeta[N-2] = psi[N-1]; cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i)
+    A pn = f[(n - 1) * LS], pc = f[(n - 2) * LS];
+    A bt = pn;
+    f[(n - 2) * LS] = bt;
+    int i = n - 2;
+    for (; i >= U; i -= U) {
+        A p[U], o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) p[u] = f[(i - 1 - u) * LS];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, p[u]));
+            bt = m_mul(cgamma[i - u], m_sub(m_mul(k.A, bt), c));
+            o[u] = bt;
+            pn = pc;
+            pc = p[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[(i - 1 - u) * LS] = o[u];
+    }
+    for (; i > 0; --i) {
+        const A pp = f[(i - 1) * LS];
+        const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+        bt = m_mul(cgamma[i], m_sub(m_mul(k.A, bt), c));
+        f[(i - 1) * LS] = bt;
+        pn = pc;
+        pc = pp;
+    }
+    // forward: psi[0] = 0; psi[i+1] = calpha[i] psi[i] + cbeta[i]; psi[N-1] = 0
+    A v = m_zero(pn);
+    double acc = 0.0;
+    i = 0;
+    for (; i + U <= n - 2; i += U) {
+        A bb[U], o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bb[u] = f[(i + u) * LS];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            o[u] = v;
+            v = m_fma(calpha[i + u], v, bb[u]);
+            if (COLS) acc += wz[i + u + 1] * m_mod2(v);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[(i + u) * LS] = o[u];
+    }
+    for (; i < n - 2; ++i) {
+        const A bi = f[i * LS];
+        f[i * LS] = v;
+        v = m_fma(calpha[i], v, bi);
+        if (COLS) acc += wz[i + 1] * m_mod2(v);
+    }
+    f[(n - 2) * LS] = v;
+    f[(n - 1) * LS] = m_zero(v);
+    return acc;
+}
+
+template <typename S, int AXIS, int L, bool COLS>
 __global__ void __launch_bounds__(256)
 k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
         const typename Arith<S>::T *__restrict__ cgamma, SweepConst<typename Arith<S>::T> k,
-        double *__restrict__ rows, const double *__restrict__ wz) {
+        double *__restrict__ colpart, const double *__restrict__ wz, const double *__restrict__ wy) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
     A *buf = reinterpret_cast<A *>(smem_raw);
@@ -214,7 +258,7 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
     const int n = AXIS == 0 ? Nx : (AXIS == 1 ? Ny : Nz);
     const int tid = threadIdx.x;
     long base;
-    int nl, xrow = 0, y0 = 0;
+    int nl, xrow = 0, y0 = 0, tile = 0;
     if constexpr (AXIS == 0) {
         const int zt = (Nz + L - 1) / L;
         const int y = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
@@ -228,7 +272,8 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
     } else {
         const int yt = (Ny + L - 1) / L;
         xrow = blockIdx.x / yt;
-        y0 = (blockIdx.x % yt) * L;
+        tile = blockIdx.x % yt;
+        y0 = tile * L;
         nl = min(L, Ny - y0);
         base = ((long)xrow * Ny + y0) * Nz;
     }
@@ -245,32 +290,14 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
         }
     }
     __syncthreads();
-    if (tid < nl) {
-        A *f = buf + tid;
-        // backward: cbeta[N-2] = psi[N-1]; cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i)
-        A pn = f[(n - 1) * LS], pc = f[(n - 2) * LS];
-        A bt = pn;
-        f[(n - 2) * LS] = bt;
-        for (int i = n - 2; i > 0; --i) {
-            const A pp = f[(i - 1) * LS];
-            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-            bt = m_mul(cgamma[i], m_sub(m_mul(k.A, bt), c));
-            f[(i - 1) * LS] = bt;
-            pn = pc;
-            pc = pp;
-        }
-        // forward: psi[0] = 0; psi[i+1] = calpha[i] psi[i] + cbeta[i]; psi[N-1] = 0
-        A v = m_zero(pn);
+    if (tid < 64) {
         double acc = 0.0;
-        for (int i = 0; i < n - 2; ++i) {
-            const A bi = f[i * LS];
-            f[i * LS] = v;
-            v = m_fma(calpha[i], v, bi);
-            if (rows) acc += wz[i + 1] * m_mod2(v);
+        if (tid < nl) acc = thomas_line<A, LS, COLS>(buf + tid, n, calpha, cgamma, k, wz);
+        if constexpr (COLS) {
+            double w = (tid < nl) ? wy[y0 + tid] * acc : 0.0;
+            w = wave_sum(w);
+            if (tid == 0) colpart[(long)tile * Nx + xrow] = w;
         }
-        f[(n - 2) * LS] = v;
-        f[(n - 1) * LS] = m_zero(v);
-        if (rows) rows[(long)(y0 + tid) * Nx + xrow] = acc;
     }
     __syncthreads();
     if constexpr (AXIS == 2) {
@@ -286,36 +313,60 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
     }
 }
 
-template <typename S, int AXIS>
+// Lines per tile: a power of two <= 64 whose LDS tile fits the budget (so
+// several workgroups share a CU while others load or store).
+static int sweep_lines(long n, size_t elem) {
+    const long budget = 40 * 1024;
+    int L = 64;
+    while (L > 4 && n * (L + 1) * (long)elem > budget) L /= 2;
+    return L;
+}
+
+static size_t arith_elem(const dbec_ctx *c) { return c->mode == DBEC_REAL ? 16 : 8; }
+
+long sweep_z_tiles(const dbec_ctx *c) {
+    const int L = sweep_lines(c->Nz, arith_elem(c));
+    return (c->Ny + L - 1) / L;
+}
+
+template <typename S, int AXIS, bool COLS>
 static hipError_t sweep_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const void *ca, const void *cg,
-                                 const SweepConst<typename Arith<S>::T> &k, double *rows, const double *wz) {
+                                 const SweepConst<typename Arith<S>::T> &k, double *colpart, const double *wz,
+                                 const double *wy) {
     using A = typename Arith<S>::T;
     const int n = AXIS == 0 ? Nx : (AXIS == 1 ? Ny : Nz);
     const int other = AXIS == 2 ? Ny : Nz;
-    // lines per tile: LDS tile <= 64 KiB, at most one wavefront of lanes
-    int L = 64;
-    while (L > 4 && (long)n * (L + 1) * (long)sizeof(A) > 65536) L /= 2;
+    const int L = sweep_lines(n, sizeof(A));
     const int tiles = (other + L - 1) / L;
     const long nblocks = (long)tiles * (AXIS == 0 ? Ny : Nx);
     const size_t lds = (size_t)n * (L + 1) * sizeof(A);
     dim3 grid((unsigned)nblocks), block(256);
     const A *cal = static_cast<const A *>(ca), *cgm = static_cast<const A *>(cg);
     switch (L) {
-    case 64: k_sweep<S, AXIS, 64><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, rows, wz); break;
-    case 32: k_sweep<S, AXIS, 32><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, rows, wz); break;
-    case 16: k_sweep<S, AXIS, 16><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, rows, wz); break;
-    case 8: k_sweep<S, AXIS, 8><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, rows, wz); break;
-    default: k_sweep<S, AXIS, 4><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, rows, wz); break;
+    case 64: k_sweep<S, AXIS, 64, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
+    case 32: k_sweep<S, AXIS, 32, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
+    case 16: k_sweep<S, AXIS, 16, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
+    case 8: k_sweep<S, AXIS, 8, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
+    default: k_sweep<S, AXIS, 4, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *rows) {
-    const auto &b = c->b;
+template <typename S, typename A>
+static hipError_t sweep_axis(hipStream_t s, S *psi, const dbec_ctx *c, int axis, const A *ca, const A *cg,
+                             const SweepConst<A> &k, double *colpart) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
+    const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
+    if (axis == 0) return sweep_dispatch<S, 0, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
+    if (axis == 1) return sweep_dispatch<S, 1, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
+    if (colpart) return sweep_dispatch<S, 2, true>(s, psi, Nx, Ny, Nz, ca, cg, k, colpart, wz, wy);
+    return sweep_dispatch<S, 2, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
+}
+
+hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colpart) {
+    const auto &b = c->b;
     const long off[3] = {0, c->Nx - 1, c->Nx - 1 + c->Ny - 1};
     const long tot = (c->Nx - 1) + (c->Ny - 1) + (c->Nz - 1);
-    const double *wz = b.wgt + c->Nx + c->Ny;
     const bool f32 = c->prec == DBEC_MIXED;
     if (c->mode == DBEC_IMAG) {
         const double *ca = b.coef + off[axis];
@@ -324,29 +375,48 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *rows
         k.A0r = c->A0r[axis][0];
         k.A = c->A[axis][0];
         k.mA = -c->A[axis][0];
-        auto run = [&](auto *psi) -> hipError_t {
-            using S = std::remove_pointer_t<decltype(psi)>;
-            if (axis == 0) return sweep_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz);
-            if (axis == 1) return sweep_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz);
-            return sweep_dispatch<S, 2>(s, psi, Nx, Ny, Nz, ca, cg, k, rows, wz);
-        };
-        return f32 ? run(static_cast<float *>(b.psi)) : run(static_cast<double *>(b.psi));
-    } else {
-        const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
-        const cpx<double> *ca = cc + off[axis];
-        const cpx<double> *cg = cc + tot + off[axis];
-        SweepConst<cpx<double>> k;
-        k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
-        k.A = {c->A[axis][0], c->A[axis][1]};
-        k.mA = {c->mA[axis][0], c->mA[axis][1]};
-        auto run = [&](auto *psi) -> hipError_t {
-            using S = std::remove_pointer_t<decltype(psi)>;
-            if (axis == 0) return sweep_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz);
-            if (axis == 1) return sweep_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz);
-            return sweep_dispatch<S, 2>(s, psi, Nx, Ny, Nz, ca, cg, k, rows, wz);
-        };
-        return f32 ? run(static_cast<cpx<float> *>(b.psi)) : run(static_cast<cpx<double> *>(b.psi));
+        return f32 ? sweep_axis(s, static_cast<float *>(b.psi), c, axis, ca, cg, k, colpart)
+                   : sweep_axis(s, static_cast<double *>(b.psi), c, axis, ca, cg, k, colpart);
     }
+    const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
+    const cpx<double> *ca = cc + off[axis];
+    const cpx<double> *cg = cc + tot + off[axis];
+    SweepConst<cpx<double>> k;
+    k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
+    k.A = {c->A[axis][0], c->A[axis][1]};
+    k.mA = {c->mA[axis][0], c->mA[axis][1]};
+    return f32 ? sweep_axis(s, static_cast<cpx<float> *>(b.psi), c, axis, ca, cg, k, colpart)
+               : sweep_axis(s, static_cast<cpx<double> *>(b.psi), c, axis, ca, cg, k, colpart);
+}
+
+// norm^2 = sum_x wx[x] sum_t colpart[t][x]; scal[NORM] = norm, scal[SCALE] = 1/norm.
+__global__ void __launch_bounds__(256)
+k_norm_cols(const double *__restrict__ colpart, int tiles, const double *__restrict__ wx, int Nx,
+            double *__restrict__ scal) {
+    __shared__ double sh[256];
+    const int tid = threadIdx.x;
+    double p = 0.0;
+    for (int x = tid; x < Nx; x += 256) {
+        double cs = 0.0;
+        for (int t = 0; t < tiles; t++) cs += colpart[(long)t * Nx + x];
+        p += wx[x] * cs;
+    }
+    sh[tid] = p;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double norm = sqrt(sh[0]);
+        scal[SC_NORM] = norm;
+        scal[SC_SCALE] = 1.0 / norm;
+    }
+}
+
+hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const double *colpart) {
+    k_norm_cols<<<1, 256, 0, s>>>(colpart, (int)sweep_z_tiles(c), c->b.wgt, (int)c->Nx, c->b.scal);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -356,11 +426,6 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *rows
 // Rows are summed by one wavefront each (lane-strided, then a fixed
 // shuffle tree), columns and the total by fixed-order loops and a fixed LDS
 // tree: the result is reproducible bit for bit from run to run.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 __device__ __forceinline__ double mod2_of(double v) { return v * v; }
 __device__ __forceinline__ double mod2_of(float v) { return (double)v * (double)v; }

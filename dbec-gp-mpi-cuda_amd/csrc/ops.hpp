@@ -5,15 +5,10 @@
 #include <type_traits>
 
 #include "internal.hpp"
+#include "nu_device.hpp"
 
 namespace dbec {
 
-struct TrapConst {
-    double half_par, g2, n2, l2;  // 0.5*par, gamma^2, nu^2, lambda^2
-};
-struct NuConst {
-    double dt, g, gd, nd;  // nd = Nx*Ny*Nz
-};
 struct MuenConst {
     double dx, dy, dz, kin, g, gd, nd;  // kin = 3 - par
 };
@@ -25,7 +20,15 @@ struct SweepConst {
 hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny, long Nz);
 hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale);
 hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale);
-hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *rows);
+// z sweep with colpart != nullptr also writes, per (y tile t, x),
+// colpart[t][x] = sum over the tile's rows of wy[y] * (Simpson z integral of psi^2).
+hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colpart);
+// number of y tiles of the z sweep (rows of colpart)
+long sweep_z_tiles(const dbec_ctx *c);
+// norm from the z sweep's column partials: scal[NORM], scal[SCALE]
+hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const double *colpart);
+TrapConst trap_const(const dbec_ctx *c);
+NuConst nu_const(const dbec_ctx *c);
 hipError_t launch_rows_mod2(hipStream_t s, const dbec_ctx *c, double *rows0, double *rowsz);
 hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren);
 // mode 0: norm and 1/norm; 1: norm and rms moments; 2: mu and E
