@@ -3,6 +3,8 @@
 // Reference aliases: imag3d.cu = src/imag3d-mpicuda/imag3d-mpicuda.cu,
 // real3d.cu = src/real3d-mpicuda/real3d-mpicuda.cu, diffint.cu =
 // src/utils/diffint.cu.
+#include <cstdlib>
+
 #include "ops.hpp"
 
 namespace dbec {
@@ -184,66 +186,110 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One line in LDS (stride LS): the reference recurrence, reads batched by U.
-template <typename A, int LS, bool COLS>
-__device__ __forceinline__ double thomas_line(A *__restrict__ f, int n, const A *__restrict__ calpha,
-                                              const A *__restrict__ cgamma, const SweepConst<A> &k,
-                                              const double *__restrict__ wz) {
-    constexpr int U = 8;
-    // backward: cbeta[N-2] = psi[N-1]; cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i)
-    A pn = f[(n - 1) * LS], pc = f[(n - 2) * LS];
-    A bt = pn;
-    f[(n - 2) * LS] = bt;
-    int i = n - 2;
-    for (; i >= U; i -= U) {
-        A p[U], o[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) p[u] = f[(i - 1 - u) * LS];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, p[u]));
-            bt = m_mul(cgamma[i - u], m_sub(m_mul(k.A, bt), c));
-            o[u] = bt;
+__device__ __forceinline__ double m_one(double) { return 1.0; }
+__device__ __forceinline__ cpx<double> m_one(cpx<double>) { return {1.0, 0.0}; }
+
+// Partitioned Thomas solve of L lines held in LDS (element e of line l at
+// buf[e * LS + l]) by 256 threads: thread (l, s) owns the steps
+// i in [ilo, ihi] of segment s of line l (P = 256 / L segments per line).
+//
+// Backward (reference: cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i), cbeta[N-2] =
+// psi[N-1]): each segment runs the same recurrence from a zero carry,
+// Q[i-1], and the product of its coefficients cgamma[m] A, P[i-1]; then
+// cbeta[i-1] = Q[i-1] + P[i-1] x with x = cbeta[ihi] handed over from the
+// segment on the right (psi[N-1] for the last one).  Forward (psi'[0] = 0,
+// psi'[i+1] = calpha[i] psi'[i] + cbeta[i]): the same with the carry psi'[ilo-1]
+// from the left.  The carries are folded through a small LDS exchange.  The
+// algebra is exact; the results differ from the sequential sweep only by
+// round-off.  cbeta[i-1] and the new psi'[i] are stored in slot i.
+template <typename A, int L, int LS, bool COLS>
+__device__ __forceinline__ void thomas_tile(A *__restrict__ buf, A *__restrict__ exa, A *__restrict__ exb,
+                                            double *__restrict__ exn, int n, int nl,
+                                            const A *__restrict__ calpha, const A *__restrict__ cgamma,
+                                            const SweepConst<A> &k, const double *__restrict__ wz,
+                                            const double *__restrict__ wy, int y0, double *colval) {
+    constexpr int P = 256 / L;
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int m = n - 2;
+    const int ilo = 1 + (s * m) / P, ihi = ((s + 1) * m) / P;
+    A *f = buf + l;
+    const A zero = m_zero(k.A), one = m_one(k.A);
+    // halos: psi[ilo-1], psi[ihi], psi[ihi+1] before any slot is overwritten
+    A h_lo = zero, p_hi = zero, h_hi = zero;
+    if (ilo <= ihi) {
+        h_lo = f[(ilo - 1) * LS];
+        p_hi = f[ihi * LS];
+        h_hi = f[(ihi + 1) * LS];
+    }
+    __syncthreads();
+    // backward, zero carry
+    A bq = zero, bp = one;
+    {
+        A pn = h_hi, pc = p_hi;
+        for (int i = ihi; i >= ilo; --i) {
+            const A pp = (i - 1 >= ilo) ? f[(i - 1) * LS] : h_lo;
+            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+            const A g = cgamma[i];
+            bq = m_mul(g, m_sub(m_mul(k.A, bq), c));
+            bp = m_mul(m_mul(g, k.A), bp);
+            f[i * LS] = bq;
             pn = pc;
-            pc = p[u];
+            pc = pp;
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) f[(i - 1 - u) * LS] = o[u];
     }
-    for (; i > 0; --i) {
-        const A pp = f[(i - 1) * LS];
-        const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-        bt = m_mul(cgamma[i], m_sub(m_mul(k.A, bt), c));
-        f[(i - 1) * LS] = bt;
-        pn = pc;
-        pc = pp;
+    exa[s * L + l] = bq;
+    exb[s * L + l] = bp;
+    __syncthreads();
+    A x = f[(n - 1) * LS];  // cbeta[N-2] = psi[N-1]
+    for (int t = P - 1; t > s; --t) x = m_add(exa[t * L + l], m_mul(exb[t * L + l], x));
+    {
+        A pr = one;
+        for (int i = ihi; i >= ilo; --i) {
+            pr = m_mul(m_mul(cgamma[i], k.A), pr);
+            f[i * LS] = m_add(f[i * LS], m_mul(pr, x));
+        }
     }
-    // forward: psi[0] = 0; psi[i+1] = calpha[i] psi[i] + cbeta[i]; psi[N-1] = 0
-    A v = m_zero(pn);
+    // forward, zero carry: slot i+1 holds cbeta[i] for i in [ilo-1, ihi-1]
+    A fy = zero, fr = one;
+    for (int i = ilo - 1; i <= ihi - 1; ++i) {
+        const A a = calpha[i];
+        fy = m_fma(a, fy, f[(i + 1) * LS]);
+        fr = m_mul(a, fr);
+        f[(i + 1) * LS] = fy;
+    }
+    __syncthreads();  // everyone has read its backward carries from exa/exb
+    exa[s * L + l] = fy;
+    exb[s * L + l] = fr;
+    __syncthreads();
+    A y = zero;  // psi'[0] = 0
+    for (int t = 0; t < s; ++t) y = m_add(exa[t * L + l], m_mul(exb[t * L + l], y));
     double acc = 0.0;
-    i = 0;
-    for (; i + U <= n - 2; i += U) {
-        A bb[U], o[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) bb[u] = f[(i + u) * LS];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            o[u] = v;
-            v = m_fma(calpha[i + u], v, bb[u]);
-            if (COLS) acc += wz[i + u + 1] * m_mod2(v);
+    {
+        A pr = one;
+        for (int i = ilo - 1; i <= ihi - 1; ++i) {
+            pr = m_mul(calpha[i], pr);
+            const A v = m_add(f[(i + 1) * LS], m_mul(pr, y));
+            f[(i + 1) * LS] = v;
+            if (COLS) acc += wz[i + 1] * m_mod2(v);
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) f[(i + u) * LS] = o[u];
     }
-    for (; i < n - 2; ++i) {
-        const A bi = f[i * LS];
-        f[i * LS] = v;
-        v = m_fma(calpha[i], v, bi);
-        if (COLS) acc += wz[i + 1] * m_mod2(v);
+    if (s == 0) f[0] = zero;
+    if (s == P - 1) f[(n - 1) * LS] = zero;
+    if constexpr (COLS) {
+        exn[s * L + l] = acc;
+        __syncthreads();
+        if (tid < 64) {
+            double w = 0.0;
+            if (tid < L && tid < nl) {
+                double row = 0.0;
+                for (int t = 0; t < P; ++t) row += exn[t * L + tid];
+                w = wy[y0 + tid] * row;
+            }
+            w = wave_sum(w);
+            if (tid == 0) *colval = w;
+        }
     }
-    f[(n - 2) * LS] = v;
-    f[(n - 1) * LS] = m_zero(v);
-    return acc;
 }
 
 template <typename S, int AXIS, int L, bool COLS>
@@ -253,9 +299,12 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
         double *__restrict__ colpart, const double *__restrict__ wz, const double *__restrict__ wy) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
-    A *buf = reinterpret_cast<A *>(smem_raw);
     constexpr int LS = (AXIS == 2) ? L + 1 : L;
     const int n = AXIS == 0 ? Nx : (AXIS == 1 ? Ny : Nz);
+    A *buf = reinterpret_cast<A *>(smem_raw);
+    A *exa = buf + n * LS;
+    A *exb = exa + 256;
+    double *exn = reinterpret_cast<double *>(exb + 256);
     const int tid = threadIdx.x;
     long base;
     int nl, xrow = 0, y0 = 0, tile = 0;
@@ -281,23 +330,19 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
     if constexpr (AXIS == 2) {
         for (int i = tid; i < n * L; i += 256) {
             const int e = i % n, l = i / n;
-            if (l < nl) buf[e * LS + l] = to_a(psi[base + (long)l * Nz + e]);
+            buf[e * LS + l] = (l < nl) ? to_a(psi[base + (long)l * Nz + e]) : m_zero(A{});
         }
     } else {
         for (int i = tid; i < n * L; i += 256) {
             const int l = i % L, e = i / L;
-            if (l < nl) buf[e * LS + l] = to_a(psi[base + e * estride + l]);
+            buf[e * LS + l] = (l < nl) ? to_a(psi[base + e * estride + l]) : m_zero(A{});
         }
     }
     __syncthreads();
-    if (tid < 64) {
-        double acc = 0.0;
-        if (tid < nl) acc = thomas_line<A, LS, COLS>(buf + tid, n, calpha, cgamma, k, wz);
-        if constexpr (COLS) {
-            double w = (tid < nl) ? wy[y0 + tid] * acc : 0.0;
-            w = wave_sum(w);
-            if (tid == 0) colpart[(long)tile * Nx + xrow] = w;
-        }
+    double colval = 0.0;
+    thomas_tile<A, L, LS, COLS>(buf, exa, exb, exn, n, nl, calpha, cgamma, k, wz, wy, y0, &colval);
+    if constexpr (COLS) {
+        if (tid == 0) colpart[(long)tile * Nx + xrow] = colval;
     }
     __syncthreads();
     if constexpr (AXIS == 2) {
@@ -313,12 +358,145 @@ k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T 
     }
 }
 
-// Lines per tile: a power of two <= 64 whose LDS tile fits the budget (so
-// several workgroups share a CU while others load or store).
+// x and y sweeps with register-resident segments.  A workgroup covers L
+// consecutive z (lanes, coalesced) of the lines of one (y) or (x) plane;
+// thread (l, s) holds the M elements [s*M, (s+1)*M) of line l in registers,
+// read straight from HBM, and runs the partitioned recurrence of thomas_tile
+// on them; only the segment carries go through LDS.  P = n / M segments.
+template <typename S, int AXIS, int L, int M>
+__global__ void __launch_bounds__(512)
+k_sweep_reg(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
+            const typename Arith<S>::T *__restrict__ cgamma, SweepConst<typename Arith<S>::T> k) {
+    using A = typename Arith<S>::T;
+    __shared__ A exa[512], exb[512], exf[512], exg[512], exl[64];
+    const int n = AXIS == 0 ? Nx : Ny;
+    const int P = n / M;
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int zt = (Nz + L - 1) / L;
+    const int pl = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
+    const int nl = min(L, Nz - z0);
+    const long base = (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + l;
+    const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
+    const bool act = l < nl;
+    const int e0 = s * M;
+    const A zero = m_zero(k.A), one = m_one(k.A);
+    A v[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) v[j] = act ? to_a(psi[base + (long)(e0 + j) * es]) : zero;
+    const A hl = (act && e0 > 0) ? to_a(psi[base + (long)(e0 - 1) * es]) : zero;
+    const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
+    // backward, zero carry; cbeta[i-1] replaces psi[i] in v[i - e0]
+    A bq = zero, bp = one;
+    {
+        A pn = hr, pc = v[M - 1];
+#pragma unroll
+        for (int j = M - 1; j >= 0; --j) {
+            const int i = e0 + j;
+            const A pp = (j > 0) ? v[j - 1] : hl;
+            if (i >= 1 && i <= n - 2) {
+                const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+                const A g = cgamma[i];
+                bq = m_mul(g, m_sub(m_mul(k.A, bq), c));
+                bp = m_mul(m_mul(g, k.A), bp);
+                v[j] = bq;
+            }
+            pn = pc;
+            pc = pp;
+        }
+    }
+    exa[s * L + l] = bq;
+    exb[s * L + l] = bp;
+    if (s == P - 1) exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
+    __syncthreads();
+    A x = exl[l];
+    for (int t = P - 1; t > s; --t) x = m_add(exa[t * L + l], m_mul(exb[t * L + l], x));
+    {
+        A pr = one;
+#pragma unroll
+        for (int j = M - 1; j >= 0; --j) {
+            const int i = e0 + j;
+            if (i >= 1 && i <= n - 2) {
+                pr = m_mul(m_mul(cgamma[i], k.A), pr);
+                v[j] = m_add(v[j], m_mul(pr, x));
+            }
+        }
+    }
+    // forward, zero carry: v[j] holds cbeta[e0 + j - 1]
+    A fy = zero, fr = one;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int i = e0 + j - 1;
+        if (i + 1 >= 1 && i + 1 <= n - 2) {
+            const A a = calpha[i];
+            fy = m_fma(a, fy, v[j]);
+            fr = m_mul(a, fr);
+            v[j] = fy;
+        }
+    }
+    exf[s * L + l] = fy;
+    exg[s * L + l] = fr;
+    __syncthreads();
+    A y = zero;
+    for (int t = 0; t < s; ++t) y = m_add(exf[t * L + l], m_mul(exg[t * L + l], y));
+    {
+        A pr = one;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const int i = e0 + j - 1;
+            if (i + 1 >= 1 && i + 1 <= n - 2) {
+                pr = m_mul(calpha[i], pr);
+                v[j] = m_add(v[j], m_mul(pr, y));
+            }
+        }
+    }
+    if (s == 0) v[0] = zero;
+    if (s == P - 1) v[M - 1] = zero;
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
+    }
+}
+
+template <typename S, int AXIS>
+static hipError_t sweep_reg_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz,
+                                     const typename Arith<S>::T *ca, const typename Arith<S>::T *cg,
+                                     const SweepConst<typename Arith<S>::T> &k) {
+    using A = typename Arith<S>::T;
+    constexpr int L = sizeof(A) == 8 ? 32 : 16;  // 256-byte rows of z per load
+    const int n = AXIS == 0 ? Nx : Ny;
+    const int M = (sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 2) ? 32 : 16;
+    const int P = n / M;
+    if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
+    const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
+    dim3 grid((unsigned)nblocks), block(L * P);
+    if constexpr (sizeof(A) == 8) {
+        if (M == 32) {
+            k_sweep_reg<S, AXIS, L, 32><<<grid, block, 0, s>>>(psi, Nx, Ny, Nz, ca, cg, k);
+            return hipGetLastError();
+        }
+    }
+    k_sweep_reg<S, AXIS, L, 16><<<grid, block, 0, s>>>(psi, Nx, Ny, Nz, ca, cg, k);
+    return hipGetLastError();
+}
+
+// LDS bytes of a sweep workgroup: the tile plus the carry exchange.
+static size_t sweep_lds(long n, int L, size_t elem) {
+    return (size_t)n * (L + 1) * elem + 2 * 256 * elem + 256 * sizeof(double);
+}
+
+// Lines per tile: the largest power of two <= 64 whose LDS fits the budget
+// (so several workgroups share a CU while others load or store), but with
+// segments of at least two steps.  DBEC_SWEEP_LDS_KB overrides the budget.
 static int sweep_lines(long n, size_t elem) {
-    const long budget = 40 * 1024;
+    static long budget = [] {
+        const char *e = getenv("DBEC_SWEEP_LDS_KB");
+        long kb = e ? atol(e) : 48;
+        return (kb > 8 && kb <= 160 ? kb : 48) * 1024;
+    }();
     int L = 64;
-    while (L > 4 && n * (L + 1) * (long)elem > budget) L /= 2;
+    while (L > 4 && (long)sweep_lds(n, L, elem) > budget) L /= 2;
+    while (L < 64 && 256 / L > (n - 2) / 2) L *= 2;
     return L;
 }
 
@@ -339,7 +517,7 @@ static hipError_t sweep_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, 
     const int L = sweep_lines(n, sizeof(A));
     const int tiles = (other + L - 1) / L;
     const long nblocks = (long)tiles * (AXIS == 0 ? Ny : Nx);
-    const size_t lds = (size_t)n * (L + 1) * sizeof(A);
+    const size_t lds = sweep_lds(n, L, sizeof(A));
     dim3 grid((unsigned)nblocks), block(256);
     const A *cal = static_cast<const A *>(ca), *cgm = static_cast<const A *>(cg);
     switch (L) {
@@ -357,8 +535,8 @@ static hipError_t sweep_axis(hipStream_t s, S *psi, const dbec_ctx *c, int axis,
                              const SweepConst<A> &k, double *colpart) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
-    if (axis == 0) return sweep_dispatch<S, 0, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
-    if (axis == 1) return sweep_dispatch<S, 1, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
+    if (axis == 0) return sweep_reg_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, k);
+    if (axis == 1) return sweep_reg_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, k);
     if (colpart) return sweep_dispatch<S, 2, true>(s, psi, Nx, Ny, Nz, ca, cg, k, colpart, wz, wy);
     return sweep_dispatch<S, 2, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
 }
