@@ -70,6 +70,22 @@ def cpu_baseline(n, h, dt, cutoff, mode, steps, threads):
     return n[0] * n[1] * n[2] * steps / t, t
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` (the runtime's blit kernel), the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def traffic_for(workload, kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (profiles/traffic_<workload>.json), or None."""
@@ -92,6 +108,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -104,6 +122,9 @@ def main():
 
     import dbecpy
 
+    if args.copy_calibration_mb:
+        copy_calibration(args.copy_calibration_mb << 20)
+        log(f"copy calibration: {args.copy_calibration_mb} MiB device-to-device")
     n, h, dt, cutoff, mode, prec = WORKLOADS[args.workload]
     lib = dbecpy.load(dbecpy.HIP_LIB)
     p = dbecpy.make_params(n[0], n[1], n[2], h, h, h, dt, 0.5, 1.0, 1.5, cutoff, device=0)

@@ -52,15 +52,7 @@ hipError_t launch_zinv(long n, const FftArgs &a) {
 }
 hipError_t launch_zinv_nu(long n, const FftArgs &a) {
     switch (n) {
-#define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a, false);
-        DBEC_FOR_EACH_FFT_LEN(DBEC_X)
-#undef DBEC_X
-    default: return hipErrorInvalidValue;
-    }
-}
-hipError_t launch_face_nu(long n, const FftArgs &a) {
-    switch (n) {
-#define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a, true);
+#define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a);
         DBEC_FOR_EACH_FFT_LEN(DBEC_X)
 #undef DBEC_X
     default: return hipErrorInvalidValue;
@@ -174,12 +166,9 @@ int run_conv_nu(dbec_ctx *c, const double *scale) {
     const double pq = (double)c->Nkz * (c->Nx / 2 + 1) * (c->Ny / 2 + 1) * real_elem(c);
     HIPCHK(c, timed(c, "xconv", 2 * spec_b + pq, [&] { return launch_xconv(c->Nx, fft_args(c, 0, nullptr)); }), "xconv");
     HIPCHK(c, timed(c, "ypass_inv", 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, nullptr)); }), "ypass");
-    const double inner = (double)(c->Nx - 1) * (c->Ny - 1) / ((double)c->Nx * c->Ny);
-    HIPCHK(c, timed(c, "zinv_nu", inner * (spec_b + 2.0 * c->N * psi_elem(c)),
-                    [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, scale)); }), "zinv_nu");
     const double nf = (double)(c->Nx + c->Ny - 1);
-    HIPCHK(c, timed(c, "face_nu", nf * c->Nkz * 2 * ce + nf * c->Nz * 2.0 * psi_elem(c),
-                    [&] { return launch_face_nu(c->Nz, fft_args(c, 2, scale)); }), "face_nu");
+    HIPCHK(c, timed(c, "zinv_nu", spec_b + 2.0 * c->N * psi_elem(c) + nf * c->Nkz * 2 * ce,
+                    [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, scale)); }), "zinv_nu");
     c->phi_valid = false;
     return DBEC_OK;
 }

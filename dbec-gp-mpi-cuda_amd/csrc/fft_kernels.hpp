@@ -16,16 +16,20 @@
 // the length N, the lines per workgroup L and the real type R of spec.
 #pragma once
 
+#include <cstdlib>
+
 #include "internal.hpp"
 
 namespace dbec {
 
-template <int N>
+template <int N, int LM = 1>
 struct FftTile {
     static constexpr int T = FftPlan<N>::T;
-    // lines per workgroup: at least 8, and at least one wavefront of threads
+    // lines per workgroup: at least 8, and at least one wavefront of threads,
+    // times the tile multiplier LM (wider tiles: longer contiguous HBM runs
+    // on the strided side, fewer workgroups per CU)
     static constexpr int L0 = (64 + T - 1) / T;
-    static constexpr int L = L0 <= 8 ? 8 : (L0 <= 16 ? 16 : (L0 <= 32 ? 32 : 64));
+    static constexpr int L = LM * (L0 <= 8 ? 8 : (L0 <= 16 ? 16 : (L0 <= 32 ? 32 : 64)));
     static constexpr int WG = L * T;
 };
 
@@ -66,11 +70,11 @@ struct PsiTraits<cpx<float>> {
 };
 
 // ---------------------------------------------------------------------------
-template <int N, typename R, typename P>
-__global__ void __launch_bounds__(FftTile<N>::WG)
+template <int N, typename R, typename P, int LM>
+__global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__restrict__ spec,
        int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -108,10 +112,10 @@ k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__re
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int DIR, typename R>
-__global__ void __launch_bounds__(FftTile<N>::WG)
+template <int N, int DIR, typename R, int LM>
+__global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -132,10 +136,12 @@ k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) 
 }
 
 // ---------------------------------------------------------------------------
-template <int N, typename R>
-__global__ void __launch_bounds__(FftTile<N>::WG)
+template <int N, typename R, int LM>
+__global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L;
+    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L;
+    constexpr int NXH = N / 2 + 1;
+    constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -144,6 +150,19 @@ k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const c
     const int y0 = (blockIdx.x % ytiles) * L;
     const int nl = min(L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
+    // potdd(|kx|, |ky|, kz) for kx <= N/2, fetched into registers up front;
+    // each value multiplies both kx and N - kx
+    const int Ny2 = Ny / 2, Nyh = Ny2 + 1;
+    const R *pk = potddq + (long)kz * NXH * Nyh;
+    R pw[QP];
+#pragma unroll
+    for (int q = 0; q < QP; ++q) {
+        const int i = tid + q * WG;
+        const int ll = i % L, kx = i / L;
+        const int y = y0 + ll;
+        const int ky = y <= Ny2 ? y : Ny - y;
+        pw[q] = (i < NXH * L && ll < nl) ? pk[kx * Nyh + ky] : R(0);
+    }
     cpx<R> *g = spec + (long)kz * N * Ny + y0;
     for (int i = tid; i < N * L; i += WG) {
         const int l = i % L, e = i / L;
@@ -152,17 +171,18 @@ k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const c
     __syncthreads();
     const int l = tid % L, t = tid / L;
     FftPlan<N>::template run<-1>(buf, LS, l, t, tw);
-    const int Ny2 = Ny / 2, Nyh = Ny2 + 1, Nxh = N / 2 + 1;
-    const R *pk = potddq + (long)kz * Nxh * Nyh;
-    for (int i = tid; i < N * L; i += WG) {
-        const int ll = i % L, e = i / L;
-        const int y = y0 + ll;
-        if (ll < nl) {
-            const int kx = e <= N / 2 ? e : N - e;
-            const int ky = y <= Ny2 ? y : Ny - y;
-            const R w = pk[kx * Nyh + ky];
-            cpx<R> v = buf[e * LS + ll];
-            buf[e * LS + ll] = {v.x * w, v.y * w};
+#pragma unroll
+    for (int q = 0; q < QP; ++q) {
+        const int i = tid + q * WG;
+        if (i < NXH * L) {
+            const int ll = i % L, kx = i / L;
+            const R w = pw[q];
+            cpx<R> v = buf[kx * LS + ll];
+            buf[kx * LS + ll] = {v.x * w, v.y * w};
+            if (kx > 0 && kx < N / 2) {
+                v = buf[(N - kx) * LS + ll];
+                buf[(N - kx) * LS + ll] = {v.x * w, v.y * w};
+            }
         }
     }
     __syncthreads();
@@ -174,10 +194,10 @@ k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const c
 }
 
 // ---------------------------------------------------------------------------
-template <int N, typename R>
-__global__ void __launch_bounds__(FftTile<N>::WG)
+template <int N, typename R, int LM>
+__global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -217,74 +237,85 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 
 // ---------------------------------------------------------------------------
 // zinv fused with calcnu (calcnu_kernel, imag3d.cu:1759-1783; real3d.cu
-// 1856-1883) for the interior rows x < Nx-1, y < Ny-1.  Within such a row the
-// face copy of calcpsidd2_kernel2b (imag3d.cu:1618-1624) takes phi[.][.][Nz-1]
-// from phi[.][.][0] of the same row, so the whole row is available on chip.
-template <int N, typename R, typename P>
-__global__ void __launch_bounds__(FftTile<N>::WG)
-k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
-          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
-    __shared__ cpx<R> buf[N * LS];
-    __shared__ cpx<R> tw[N];
-    const int tid = threadIdx.x;
-    const int ytiles = (Ny + 2 * L - 1) / (2 * L);
-    const int x = blockIdx.x / ytiles;  // grid covers x < Nx - 1
-    const int y0 = (blockIdx.x % ytiles) * 2 * L;
-    const int nrows = min(2 * L, Ny - y0);
-    stage_twiddles(tw, twg, N, tid, WG);
-    const int nkz = N / 2 + 1;
-    for (int i = tid; i < nkz * L; i += WG) {
-        const int p = i % L, k = i / L;
-        const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
-        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-        if (2 * p < nrows) a = spec[base];
-        if (2 * p + 1 < nrows) b = spec[base + 1];
-        if (k == 0 || k == N / 2) {
-            a.y = R(0);
-            b.y = R(0);
-        }
-        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
-        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
-    }
-    __syncthreads();
-    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
-    const double s = scale ? *scale : 1.0;
-    const double x2 = sq[x];
-    P *dst = psi + ((long)x * Ny + y0) * N;
-    for (int i = tid; i < 2 * L * N; i += WG) {
-        const int z = i % N, row = i / N;
-        const int y = y0 + row;
-        if (row < nrows && y < Ny - 1) {
-            const cpx<R> c = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
-            const double phi = (double)((row & 1) ? c.y : c.x);
-            const double pot = trap(tc, x2, sq[Nx + y], sq[Nx + Ny + z]);
-            const long n = (long)row * N + z;
-            dst[n] = nu_apply(dst[n], s, phi, pot, nc);
-        }
-    }
-}
-
-// calcnu on the face rows, whose phi comes from other rows after the
-// periodic face copies (calcpsidd2_kernel2a/2b, imag3d.cu:1591-1625):
+// 1856-1883).  Blocks [0, nmain) take the interior rows x < Nx-1, y < Ny-1:
+// within such a row the face copy of calcpsidd2_kernel2b (imag3d.cu
+// 1618-1624) takes phi[.][.][Nz-1] from phi[.][.][0] of the same row, so the
+// row's potential never leaves the chip; their psi is fetched into registers
+// before the transform so its latency overlaps the spectrum load.  The
+// remaining blocks take the face rows, whose phi comes from other rows after
+// the periodic face copies (calcpsidd2_kernel2a/2b, imag3d.cu:1591-1625):
 //   row (Nx-1, y):          z < Nz-1 and y < Ny-1 from row (0, y), else own
 //   row (x, Ny-1), x<Nx-1:  z < Nz-1 from row (x, 0), z = Nz-1 own.
-// Each face row is transformed together with its source row as the real and
-// imaginary parts of one complex line.  Face f < Ny is (Nx-1, f); face
-// f >= Ny is (f - Ny, Ny-1).
-template <int N, typename R, typename P>
-__global__ void __launch_bounds__(FftTile<N>::WG)
-k_face_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
-          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N>::L, T = FftTile<N>::T, WG = FftTile<N>::WG, LS = L + 1;
+// A face row is transformed together with its source row as the real and
+// imaginary parts of one complex line; face f < Ny is (Nx-1, f), face f >= Ny
+// is (f - Ny, Ny-1).
+template <int N, typename R, typename P, int LM>
+__global__ void __launch_bounds__((FftTile<N, LM>::WG))
+k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
+          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int nmain,
+          const cpx<R> *__restrict__ twg) {
+    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int Q = (2 * L * N + WG - 1) / WG;  // psi elements per thread (interior rows)
+    constexpr bool PREFETCH = sizeof(P) <= 8;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
-    const int nface = Ny + Nx - 1;
-    const int f0 = blockIdx.x * L;
-    const int nl = min(L, nface - f0);
-    stage_twiddles(tw, twg, N, tid, WG);
     const int nkz = N / 2 + 1;
+    const double s = scale ? *scale : 1.0;
+    stage_twiddles(tw, twg, N, tid, WG);
+    if ((int)blockIdx.x < nmain) {
+        const int ytiles = (Ny + 2 * L - 1) / (2 * L);
+        const int x = blockIdx.x / ytiles;  // x < Nx - 1
+        const int y0 = (blockIdx.x % ytiles) * 2 * L;
+        const int nrows = min(2 * L, Ny - y0);
+        P *dst = psi + ((long)x * Ny + y0) * N;
+        P pv[PREFETCH ? Q : 1];
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int i = tid + q * WG;
+                const int z = i % N, row = i / N;
+                if (i < 2 * L * N && row < nrows && y0 + row < Ny - 1) pv[q] = dst[(long)row * N + z];
+            }
+        }
+        for (int i = tid; i < nkz * L; i += WG) {
+            const int p = i % L, k = i / L;
+            const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+            cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+            if (2 * p < nrows) a = spec[base];
+            if (2 * p + 1 < nrows) b = spec[base + 1];
+            if (k == 0 || k == N / 2) {
+                a.y = R(0);
+                b.y = R(0);
+            }
+            buf[k * LS + p] = {a.x - b.y, a.y + b.x};
+            if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+        }
+        __syncthreads();
+        FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+        const double x2 = sq[x];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int i = tid + q * WG;
+            const int z = i % N, row = i / N;
+            const int y = y0 + row;
+            if (i < 2 * L * N && row < nrows && y < Ny - 1) {
+                const cpx<R> c = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
+                const double phi = (double)((row & 1) ? c.y : c.x);
+                const double pot = trap(tc, x2, sq[Nx + y], sq[Nx + Ny + z]);
+                const long n = (long)row * N + z;
+                P v;
+                if constexpr (PREFETCH) v = pv[q];
+                else v = dst[n];
+                dst[n] = nu_apply(v, s, phi, pot, nc);
+            }
+        }
+        return;
+    }
+    // face rows
+    const int nface = Ny + Nx - 1;
+    const int f0 = ((int)blockIdx.x - nmain) * L;
+    const int nl = min(L, nface - f0);
     for (int i = tid; i < nkz * L; i += WG) {
         const int p = i % L, k = i / L;
         cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
@@ -305,7 +336,6 @@ k_face_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__
     }
     __syncthreads();
     FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
-    const double s = scale ? *scale : 1.0;
     for (int i = tid; i < L * N; i += WG) {
         const int z = i % N, p = i / N;
         if (p < nl) {
@@ -324,96 +354,120 @@ k_face_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__
 }
 
 // ---------------------------------------------------------------------------
-// Launchers for one length.
+// Launchers for one length.  The z forward pass runs with tiles twice as
+// wide (measured faster at 256^3); DBEC_FFT_LM_ZFWD=1 selects the narrow one.
+inline int fft_lm(const char *name, int dflt) {
+    const char *e = getenv(name);
+    if (!e) return dflt;
+    int v = atoi(e);
+    return (v == 1 || v == 2) ? v : dflt;
+}
+
 template <int N>
 struct FftLaunch {
-    using Tl = FftTile<N>;
-    static hipError_t zfwd(const FftArgs &a) {
+    template <int LM>
+    static hipError_t zfwd_t(const FftArgs &a) {
+        using Tl = FftTile<N, LM>;
         const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
         dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
         if (!a.fp32) {
             auto *spec = static_cast<cpx<double> *>(a.spec);
             auto *tw = static_cast<const cpx<double> *>(a.tw);
             if (a.cpsi)
-                k_zfwd<N, double, cpx<double>><<<grid, block, 0, a.stream>>>(
+                k_zfwd<N, double, cpx<double>, LM><<<grid, block, 0, a.stream>>>(
                     static_cast<const cpx<double> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
             else
-                k_zfwd<N, double, double><<<grid, block, 0, a.stream>>>(
+                k_zfwd<N, double, double, LM><<<grid, block, 0, a.stream>>>(
                     static_cast<const double *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
         } else {
             auto *spec = static_cast<cpx<float> *>(a.spec);
             auto *tw = static_cast<const cpx<float> *>(a.tw);
             if (a.cpsi)
-                k_zfwd<N, float, cpx<float>><<<grid, block, 0, a.stream>>>(
+                k_zfwd<N, float, cpx<float>, LM><<<grid, block, 0, a.stream>>>(
                     static_cast<const cpx<float> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
             else
-                k_zfwd<N, float, float><<<grid, block, 0, a.stream>>>(
+                k_zfwd<N, float, float, LM><<<grid, block, 0, a.stream>>>(
                     static_cast<const float *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
         }
         return hipGetLastError();
     }
-    static hipError_t ypass(int dir, const FftArgs &a) {
+    static hipError_t zfwd(const FftArgs &a) {
+        static const int lm = fft_lm("DBEC_FFT_LM_ZFWD", 2);
+        return lm == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
+    }
+
+    template <int LM>
+    static hipError_t ypass_t(int dir, const FftArgs &a) {
+        using Tl = FftTile<N, LM>;
         const long nlines = a.Nkz * a.Nx;
         dim3 grid((unsigned)((nlines + Tl::L - 1) / Tl::L)), block(Tl::WG);
         if (!a.fp32) {
             auto *spec = static_cast<cpx<double> *>(a.spec);
             auto *tw = static_cast<const cpx<double> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, double><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
-            else k_ypass<N, +1, double><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, double, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            else k_ypass<N, +1, double, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
         } else {
             auto *spec = static_cast<cpx<float> *>(a.spec);
             auto *tw = static_cast<const cpx<float> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, float><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
-            else k_ypass<N, +1, float><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, float, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            else k_ypass<N, +1, float, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
         }
         return hipGetLastError();
     }
-    static hipError_t xconv(const FftArgs &a) {
+    static hipError_t ypass(int dir, const FftArgs &a) { return ypass_t<1>(dir, a); }
+
+    template <int LM>
+    static hipError_t xconv_t(const FftArgs &a) {
+        using Tl = FftTile<N, LM>;
         const int ytiles = (int)((a.Ny + Tl::L - 1) / Tl::L);
         dim3 grid((unsigned)(a.Nkz * ytiles)), block(Tl::WG);
         if (!a.fp32)
-            k_xconv<N, double><<<grid, block, 0, a.stream>>>(static_cast<cpx<double> *>(a.spec),
-                                                             static_cast<const double *>(a.potddq), (int)a.Ny,
-                                                             static_cast<const cpx<double> *>(a.tw));
+            k_xconv<N, double, LM><<<grid, block, 0, a.stream>>>(static_cast<cpx<double> *>(a.spec),
+                                                                 static_cast<const double *>(a.potddq), (int)a.Ny,
+                                                                 static_cast<const cpx<double> *>(a.tw));
         else
-            k_xconv<N, float><<<grid, block, 0, a.stream>>>(static_cast<cpx<float> *>(a.spec),
-                                                            static_cast<const float *>(a.potddq), (int)a.Ny,
-                                                            static_cast<const cpx<float> *>(a.tw));
+            k_xconv<N, float, LM><<<grid, block, 0, a.stream>>>(static_cast<cpx<float> *>(a.spec),
+                                                                static_cast<const float *>(a.potddq), (int)a.Ny,
+                                                                static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
     }
+    static hipError_t xconv(const FftArgs &a) { return xconv_t<1>(a); }
+
     static hipError_t zinv(const FftArgs &a) {
+        using Tl = FftTile<N, 1>;
         const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
         dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
         if (!a.fp32)
-            k_zinv<N, double><<<grid, block, 0, a.stream>>>(static_cast<const cpx<double> *>(a.spec),
-                                                            static_cast<double *>(a.phi), (int)a.Nx, (int)a.Ny,
-                                                            static_cast<const cpx<double> *>(a.tw));
+            k_zinv<N, double, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<double> *>(a.spec),
+                                                               static_cast<double *>(a.phi), (int)a.Nx, (int)a.Ny,
+                                                               static_cast<const cpx<double> *>(a.tw));
         else
-            k_zinv<N, float><<<grid, block, 0, a.stream>>>(static_cast<const cpx<float> *>(a.spec),
-                                                           static_cast<float *>(a.phi), (int)a.Nx, (int)a.Ny,
-                                                           static_cast<const cpx<float> *>(a.tw));
+            k_zinv<N, float, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<float> *>(a.spec),
+                                                              static_cast<float *>(a.phi), (int)a.Nx, (int)a.Ny,
+                                                              static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
     }
-    template <typename P, typename R>
-    static hipError_t fused_t(const FftArgs &a, bool faces) {
+
+    template <typename P, typename R, int LM>
+    static hipError_t fused_t(const FftArgs &a) {
+        using Tl = FftTile<N, LM>;
         const auto *spec = static_cast<const cpx<R> *>(a.spec);
         auto *psi = static_cast<P *>(a.psi);
         const auto *tw = static_cast<const cpx<R> *>(a.tw);
-        if (!faces) {
-            const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
-            dim3 grid((unsigned)((a.Nx - 1) * ytiles)), block(Tl::WG);
-            k_zinv_nu<N, R, P><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx, (int)a.Ny, tw);
-        } else {
-            const long nface = a.Ny + a.Nx - 1;
-            dim3 grid((unsigned)((nface + Tl::L - 1) / Tl::L)), block(Tl::WG);
-            k_face_nu<N, R, P><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx, (int)a.Ny, tw);
-        }
+        const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
+        const int nmain = (int)((a.Nx - 1) * ytiles);
+        const int nfaceb = (int)((a.Ny + a.Nx - 1 + Tl::L - 1) / Tl::L);
+        dim3 grid((unsigned)(nmain + nfaceb)), block(Tl::WG);
+        k_zinv_nu<N, R, P, LM><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx,
+                                                             (int)a.Ny, nmain, tw);
         return hipGetLastError();
     }
-    static hipError_t fused(const FftArgs &a, bool faces) {
-        if (!a.fp32) return a.cpsi ? fused_t<cpx<double>, double>(a, faces) : fused_t<double, double>(a, faces);
-        return a.cpsi ? fused_t<cpx<float>, float>(a, faces) : fused_t<float, float>(a, faces);
+    template <int LM>
+    static hipError_t fused_lm(const FftArgs &a) {
+        if (!a.fp32) return a.cpsi ? fused_t<cpx<double>, double, LM>(a) : fused_t<double, double, LM>(a);
+        return a.cpsi ? fused_t<cpx<float>, float, LM>(a) : fused_t<float, float, LM>(a);
     }
+    static hipError_t fused(const FftArgs &a) { return fused_lm<1>(a); }
 };
 
 }  // namespace dbec

@@ -91,10 +91,9 @@ hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C alo
 hipError_t launch_ypass(long n, int dir, const FftArgs &a);
 hipError_t launch_xconv(long n, const FftArgs &a);   // x FFT, * potdd, inverse x FFT
 hipError_t launch_zinv(long n, const FftArgs &a);    // spec -> phi (C2R along z)
-// C2R along z fused with calcnu for the rows x < Nx-1, y < Ny-1 (phi never
-// leaves the chip), and the same for the face rows x = Nx-1 or y = Ny-1 with
-// the reference's periodic face copies of phi.
+// C2R along z fused with calcnu: the interior rows with phi kept on chip and
+// the face rows x = Nx-1 or y = Ny-1 with the reference's periodic face
+// copies of phi, in one launch.
 hipError_t launch_zinv_nu(long n, const FftArgs &a);
-hipError_t launch_face_nu(long n, const FftArgs &a);
 
 }  // namespace dbec

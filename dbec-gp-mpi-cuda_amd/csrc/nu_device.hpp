@@ -18,7 +18,13 @@ struct TrapConst {
 };
 struct NuConst {
     double dt, g, gd, nd;  // nd = Nx*Ny*Nz
+    double inv_nd;         // 1/nd, used when nd is a power of two (then phi*inv_nd == phi/nd exactly)
+    int nd_pow2;
 };
+
+__device__ __forceinline__ double div_nd(double phi, const NuConst &nc) {
+    return nc.nd_pow2 ? phi * nc.inv_nd : phi / nc.nd;
+}
 
 // 0.5*par*(g^2 x^2 + n^2 y^2 + l^2 z^2) in the reference's evaluation order
 // with no contraction, i.e. the number the reference's pot table holds.
@@ -30,7 +36,7 @@ __device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2
 // imaginary time: psi * exp(-dt (pot + g psi^2 + gd phi / N))
 __device__ __forceinline__ double nu_imag(double psitmp, double phi, double pot, const NuConst &nc) {
     const double psi2lin = psitmp * psitmp * nc.g;
-    const double psidd2lin = (phi / nc.nd) * nc.gd;
+    const double psidd2lin = div_nd(phi, nc) * nc.gd;
     const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
     return psitmp * exp(-pottmp);
 }
@@ -39,7 +45,7 @@ __device__ __forceinline__ double nu_imag(double psitmp, double phi, double pot,
 __device__ __forceinline__ cpx<double> nu_real(cpx<double> v, double phi, double pot, const NuConst &nc) {
     const double a = hypot(v.x, v.y);
     const double psi2lin = a * a * nc.g;
-    const double psidd2lin = (phi / nc.nd) * nc.gd;
+    const double psidd2lin = div_nd(phi, nc) * nc.gd;
     const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
     double sn, cs;
     sincos(-pottmp, &sn, &cs);

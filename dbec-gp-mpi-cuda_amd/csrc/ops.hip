@@ -75,7 +75,10 @@ NuConst nu_const(const dbec_ctx *c) {
     nc.dt = c->p.dt;
     nc.g = c->g;
     nc.gd = c->gd;
-    nc.nd = (double)(c->Nx * c->Ny * c->Nz);
+    const long nd = c->Nx * c->Ny * c->Nz;
+    nc.nd = (double)nd;
+    nc.inv_nd = 1.0 / (double)nd;
+    nc.nd_pow2 = (nd & (nd - 1)) == 0;
     return nc;
 }
 
