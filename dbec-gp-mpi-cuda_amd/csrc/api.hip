@@ -277,7 +277,8 @@ int setup(dbec_ctx *c) {
     const double h2[3] = {p.dx * p.dx, p.dy * p.dy, p.dz * p.dz};
     const long n3[3] = {Nx, Ny, Nz};
     const long tot = (Nx - 1) + (Ny - 1) + (Nz - 1);
-    std::vector<double> coef((cplx ? 2 : 1) * 2 * tot);
+    // [calpha | cgamma | cgamma * A] per axis x, y, z
+    std::vector<double> coef((cplx ? 2 : 1) * 3 * tot);
     long off = 0;
     for (int a = 0; a < 3; a++) {
         const long n = n3[a];
@@ -297,6 +298,7 @@ int setup(dbec_ctx *c) {
             for (long i = 0; i < n - 1; i++) {
                 coef[off + i] = al[i];
                 coef[tot + off + i] = ga[i];
+                coef[2 * tot + off + i] = ga[i] * A;
             }
             c->h_calpha[a] = al;
             c->h_cgamma[a] = ga;
@@ -320,6 +322,9 @@ int setup(dbec_ctx *c) {
                 coef[2 * (off + i) + 1] = al[i].im;
                 coef[2 * (tot + off + i)] = ga[i].re;
                 coef[2 * (tot + off + i) + 1] = ga[i].im;
+                const Cx gA = cx_mul(ga[i], A);
+                coef[2 * (2 * tot + off + i)] = gA.re;
+                coef[2 * (2 * tot + off + i) + 1] = gA.im;
                 c->h_calpha[a][2 * i] = al[i].re;
                 c->h_calpha[a][2 * i + 1] = al[i].im;
                 c->h_cgamma[a][2 * i] = ga[i].re;

@@ -21,6 +21,26 @@ struct alignas(2 * sizeof(R)) cpx {
     R x, y;
 };
 
+// 16-byte packets of storage elements: one global_load/store_dwordx4 per lane.
+// Staging loops load every packet of a tile into registers first (addresses
+// clamped into the array instead of branching), so a wavefront has all its
+// loads in flight at once, and only then write the tile to LDS.
+template <typename T>
+struct Pk {
+    static constexpr int n = sizeof(T) >= 16 ? 1 : 16 / (int)sizeof(T);
+    struct alignas(16) type {
+        T e[n];
+    };
+};
+template <typename T>
+__device__ __forceinline__ typename Pk<T>::type ld_pk(const T *p) {
+    return *reinterpret_cast<const typename Pk<T>::type *>(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_pk(T *p, const typename Pk<T>::type &v) {
+    *reinterpret_cast<typename Pk<T>::type *>(p) = v;
+}
+
 template <typename R>
 __device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename R>

@@ -74,7 +74,7 @@ template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__restrict__ spec,
        int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -84,22 +84,45 @@ k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__re
     const int nrows = min(2 * L, Ny - y0);
     const double s = scale ? *scale : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
+    // Rows 2p and 2p+1 of the tile become the real and imaginary parts of
+    // line p.  A thread takes VE consecutive z of both rows, so its LDS
+    // writes are whole complex slots (stride LS: conflict-free); all of its
+    // loads are issued before the first LDS write.
+    constexpr int VE = sizeof(P) == 4 ? 2 : 1, NVR = N / VE, KV = L * NVR / WG;
+    static_assert((L * NVR) % WG == 0, "zfwd tile");
+    struct alignas(VE * sizeof(P)) Pv {
+        P e[VE];
+    };
     const P *src = psi + ((long)x * Ny + y0) * N;
-    for (int i = tid; i < 2 * L * N; i += WG) {
-        const int z = i % N, row = i / N;
-        R v = R(0);
-        if (row < nrows) v = (R)PsiTraits<P>::mod2(src[(long)row * N + z], s);
-        R *slot = reinterpret_cast<R *>(&buf[z * LS + (row >> 1)]);
-        slot[row & 1] = v;
+    Pv ra[KV], rb[KV];
+#pragma unroll
+    for (int k = 0; k < KV; ++k) {
+        const int iv = tid + k * WG;
+        const int p = iv / NVR, zv = iv % NVR;
+        ra[k] = *reinterpret_cast<const Pv *>(src + (long)min(2 * p, nrows - 1) * N + zv * VE);
+        rb[k] = *reinterpret_cast<const Pv *>(src + (long)min(2 * p + 1, nrows - 1) * N + zv * VE);
+    }
+#pragma unroll
+    for (int k = 0; k < KV; ++k) {
+        const int iv = tid + k * WG;
+        const int p = iv / NVR, zv = iv % NVR;
+#pragma unroll
+        for (int j = 0; j < VE; ++j) {
+            const R va = 2 * p < nrows ? (R)PsiTraits<P>::mod2(ra[k].e[j], s) : R(0);
+            const R vb = 2 * p + 1 < nrows ? (R)PsiTraits<P>::mod2(rb[k].e[j], s) : R(0);
+            buf[(zv * VE + j) * LS + p] = {va, vb};
+        }
     }
     __syncthreads();
     FftPlan<N>::template run<-1>(buf, LS, tid % L, tid / L, tw);
     // split Z = A + iB into the spectra of the two real rows:
     //   A[k] = (Z[k] + conj Z[N-k]) / 2,  B[k] = (Z[k] - conj Z[N-k]) / (2i)
-    const int nkz = N / 2 + 1;
-    for (int i = tid; i < nkz * 2 * L; i += WG) {
+    constexpr int nkz = N / 2 + 1;
+#pragma unroll
+    for (int q = 0; q < (nkz * 2 * L + WG - 1) / WG; ++q) {
+        const int i = tid + q * WG;
         const int row = i % (2 * L), k = i / (2 * L);
-        if (row < nrows) {
+        if (k < nkz && row < nrows) {
             const int p = row >> 1;
             const cpx<R> z = buf[k * LS + p];
             const cpx<R> w = buf[((N - k) % N) * LS + p];
@@ -115,7 +138,7 @@ k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__re
 template <int N, int DIR, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -123,15 +146,25 @@ k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) 
     const long line0 = (long)blockIdx.x * L;
     const int nl = (int)min((long)L, nlines - line0);
     cpx<R> *g = spec + line0 * N;
-    for (int i = tid; i < nl * N; i += WG) {
-        const int e = i % N, l = i / N;
-        buf[e * LS + l] = g[i];
+    constexpr int K = L * N / WG;
+    static_assert((L * N) % WG == 0, "ypass tile");
+    cpx<R> r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
+        r[k] = g[i < nl * N ? i : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
+        buf[(i % N) * LS + i / N] = r[k];
     }
     __syncthreads();
     FftPlan<N>::template run<DIR>(buf, LS, tid % L, tid / L, tw);
-    for (int i = tid; i < nl * N; i += WG) {
-        const int e = i % N, l = i / N;
-        g[i] = buf[e * LS + l];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
+        if (i < nl * N) g[i] = buf[(i % N) * LS + i / N];
     }
 }
 
@@ -139,7 +172,7 @@ k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) 
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L;
     constexpr int NXH = N / 2 + 1;
     constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
@@ -164,9 +197,19 @@ k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const c
         pw[q] = (i < NXH * L && ll < nl) ? pk[kx * Nyh + ky] : R(0);
     }
     cpx<R> *g = spec + (long)kz * N * Ny + y0;
-    for (int i = tid; i < N * L; i += WG) {
-        const int l = i % L, e = i / L;
-        if (l < nl) buf[e * LS + l] = g[(long)e * Ny + l];
+    constexpr int K = L * N / WG;
+    static_assert((L * N) % WG == 0, "xconv tile");
+    cpx<R> r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
+        const int ll = i % L, e = i / L;
+        r[k] = g[(long)e * Ny + (ll < nl ? ll : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
+        buf[(i / L) * LS + i % L] = r[k];
     }
     __syncthreads();
     const int l = tid % L, t = tid / L;
@@ -187,17 +230,60 @@ k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const c
     }
     __syncthreads();
     FftPlan<N>::template run<+1>(buf, LS, l, t, tw);
-    for (int i = tid; i < N * L; i += WG) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid + k * WG;
         const int ll = i % L, e = i / L;
         if (ll < nl) g[(long)e * Ny + ll] = buf[e * LS + ll];
     }
 }
 
 // ---------------------------------------------------------------------------
+// Inverse z transform of two real rows per complex line: line p holds
+//   Z[k] = A[k] + i B[k] (k <= N/2) and Z[N-k] = conj A[k] + i conj B[k],
+// A, B the half spectra of the rows; the imaginary parts of the
+// self-conjugate bins k = 0, N/2 are dropped.  SpecIdx(p, k, &ia, &ib) sets
+// the spec indices of A and B and returns which of them exist (bit 0: A,
+// bit 1: B); a missing one reads as zero.  All loads are issued before the
+// first LDS write.
+template <int N, int L, int WG, int LS, typename R, typename SpecIdx>
+__device__ __forceinline__ void load_hermitian_pairs(cpx<R> *buf, const cpx<R> *__restrict__ spec, int tid,
+                                                     const SpecIdx &idx) {
+    constexpr int nkz = N / 2 + 1;
+    constexpr int K = (nkz * L + WG - 1) / WG;
+    cpx<R> ra[K], rb[K];
+    int m[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int i = tid + q * WG;
+        const int p = i % L, k = i / L;
+        long ia = 0, ib = 0;
+        m[q] = i < nkz * L ? idx(p, k, ia, ib) : 0;
+        ra[q] = spec[(m[q] & 1) ? ia : 0];
+        rb[q] = spec[(m[q] & 2) ? ib : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int i = tid + q * WG;
+        const int p = i % L, k = i / L;
+        if (i < nkz * L) {
+            cpx<R> a = ra[q], b = rb[q];
+            if (!(m[q] & 1)) a = {R(0), R(0)};
+            if (!(m[q] & 2)) b = {R(0), R(0)};
+            if (k == 0 || k == N / 2) {
+                a.y = R(0);
+                b.y = R(0);
+            }
+            buf[k * LS + p] = {a.x - b.y, a.y + b.x};
+            if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+        }
+    }
+}
+
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
@@ -206,26 +292,18 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
     const int y0 = (blockIdx.x % ytiles) * 2 * L;
     const int nrows = min(2 * L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
-    // Z[k] = A[k] + i B[k] (k <= N/2) and Z[N-k] = conj A[k] + i conj B[k];
-    // the imaginary parts of the self-conjugate bins k = 0, N/2 are dropped.
-    const int nkz = N / 2 + 1;
-    for (int i = tid; i < nkz * L; i += WG) {
-        const int p = i % L, k = i / L;
-        const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
-        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-        if (2 * p < nrows) a = spec[base];
-        if (2 * p + 1 < nrows) b = spec[base + 1];
-        if (k == 0 || k == N / 2) {
-            a.y = R(0);
-            b.y = R(0);
-        }
-        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
-        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
-    }
+    load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+        ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+        ib = ia + 1;
+        return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
+    });
     __syncthreads();
     FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
     R *dst = phi + ((long)x * Ny + y0) * N;
-    for (int i = tid; i < 2 * L * N; i += WG) {
+    constexpr int Q = 2 * L * N / WG;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int i = tid + q * WG;
         const int z = i % N, row = i / N;
         if (row < nrows) {
             const cpx<R> c = buf[z * LS + (row >> 1)];
@@ -237,118 +315,135 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 
 // ---------------------------------------------------------------------------
 // zinv fused with calcnu (calcnu_kernel, imag3d.cu:1759-1783; real3d.cu
-// 1856-1883).  Blocks [0, nmain) take the interior rows x < Nx-1, y < Ny-1:
-// within such a row the face copy of calcpsidd2_kernel2b (imag3d.cu
-// 1618-1624) takes phi[.][.][Nz-1] from phi[.][.][0] of the same row, so the
-// row's potential never leaves the chip; their psi is fetched into registers
-// before the transform so its latency overlaps the spectrum load.  The
-// remaining blocks take the face rows, whose phi comes from other rows after
-// the periodic face copies (calcpsidd2_kernel2a/2b, imag3d.cu:1591-1625):
+// 1856-1883).  Blocks [0, nfaceb) take the face rows x = Nx-1 or y = Ny-1,
+// whose phi comes from other rows after the periodic face copies
+// (calcpsidd2_kernel2a/2b, imag3d.cu:1591-1625):
 //   row (Nx-1, y):          z < Nz-1 and y < Ny-1 from row (0, y), else own
 //   row (x, Ny-1), x<Nx-1:  z < Nz-1 from row (x, 0), z = Nz-1 own.
 // A face row is transformed together with its source row as the real and
 // imaginary parts of one complex line; face f < Ny is (Nx-1, f), face f >= Ny
-// is (f - Ny, Ny-1).
+// is (f - Ny, Ny-1).  They go first so that they overlap the interior
+// blocks instead of trailing them.  The remaining blocks take the interior
+// rows x < Nx-1, y < Ny-1: within such a row the face copy of
+// calcpsidd2_kernel2b (imag3d.cu:1618-1624) takes phi[.][.][Nz-1] from
+// phi[.][.][0] of the same row, so the row's potential never leaves the chip.
+// psi of 8 bytes or less per element is fetched into registers before the
+// transform, so its latency overlaps the spectrum load; complex FP64 psi is
+// fetched after it in batches of 8.  The squared coordinates of the trap
+// potential are staged in LDS.
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
-          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int nmain,
+          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int nfaceb,
           const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, T = FftTile<N, LM>::T, WG = FftTile<N, LM>::WG, LS = L + 1;
-    constexpr int Q = (2 * L * N + WG - 1) / WG;  // psi elements per thread (interior rows)
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int Q = 2 * L * N / WG;  // psi elements per thread (interior rows)
     constexpr bool PREFETCH = sizeof(P) <= 8;
+    constexpr int C = PREFETCH ? Q : (Q < 8 ? Q : 8);  // psi batch
+    static_assert((2 * L * N) % WG == 0 && Q % C == 0, "zinv_nu tile");
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
+    __shared__ double sqz[N];
+    __shared__ double sqr[2 * L];  // interior: y^2 of the tile's rows; faces: x^2 | y^2 of each face row
     const int tid = threadIdx.x;
-    const int nkz = N / 2 + 1;
     const double s = scale ? *scale : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
-    if ((int)blockIdx.x < nmain) {
+    for (int i = tid; i < N; i += WG) sqz[i] = sq[Nx + Ny + i];
+    if ((int)blockIdx.x >= nfaceb) {
+        const int b = (int)blockIdx.x - nfaceb;
         const int ytiles = (Ny + 2 * L - 1) / (2 * L);
-        const int x = blockIdx.x / ytiles;  // x < Nx - 1
-        const int y0 = (blockIdx.x % ytiles) * 2 * L;
+        const int x = b / ytiles;  // x < Nx - 1
+        const int y0 = (b % ytiles) * 2 * L;
         const int nrows = min(2 * L, Ny - y0);
         P *dst = psi + ((long)x * Ny + y0) * N;
-        P pv[PREFETCH ? Q : 1];
+        if (tid < 2 * L) sqr[tid] = sq[Nx + min(y0 + tid, Ny - 1)];
+        P pv[C];
         if constexpr (PREFETCH) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int i = tid + q * WG;
-                const int z = i % N, row = i / N;
-                if (i < 2 * L * N && row < nrows && y0 + row < Ny - 1) pv[q] = dst[(long)row * N + z];
+                pv[q] = dst[(long)min(i / N, nrows - 1) * N + i % N];
             }
         }
-        for (int i = tid; i < nkz * L; i += WG) {
-            const int p = i % L, k = i / L;
-            const long base = ((long)k * Nx + x) * Ny + y0 + 2 * p;
-            cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-            if (2 * p < nrows) a = spec[base];
-            if (2 * p + 1 < nrows) b = spec[base + 1];
-            if (k == 0 || k == N / 2) {
-                a.y = R(0);
-                b.y = R(0);
-            }
-            buf[k * LS + p] = {a.x - b.y, a.y + b.x};
-            if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
-        }
+        load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+            ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+            ib = ia + 1;
+            return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
+        });
         __syncthreads();
         FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
         const double x2 = sq[x];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int i = tid + q * WG;
-            const int z = i % N, row = i / N;
-            const int y = y0 + row;
-            if (i < 2 * L * N && row < nrows && y < Ny - 1) {
-                const cpx<R> c = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
-                const double phi = (double)((row & 1) ? c.y : c.x);
-                const double pot = trap(tc, x2, sq[Nx + y], sq[Nx + Ny + z]);
-                const long n = (long)row * N + z;
-                P v;
-                if constexpr (PREFETCH) v = pv[q];
-                else v = dst[n];
-                dst[n] = nu_apply(v, s, phi, pot, nc);
+        for (int q0 = 0; q0 < Q; q0 += C) {
+            if constexpr (!PREFETCH) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int i = tid + (q0 + c) * WG;
+                    pv[c] = dst[(long)min(i / N, nrows - 1) * N + i % N];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int i = tid + (q0 + c) * WG;
+                const int z = i % N, row = i / N;
+                if (row < nrows && y0 + row < Ny - 1) {
+                    const cpx<R> cv = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
+                    const double phi = (double)((row & 1) ? cv.y : cv.x);
+                    const double pot = trap(tc, x2, sqr[row], sqz[z]);
+                    dst[(long)row * N + z] = nu_apply(pv[PREFETCH ? q0 + c : c], s, phi, pot, nc);
+                }
             }
         }
         return;
     }
     // face rows
     const int nface = Ny + Nx - 1;
-    const int f0 = ((int)blockIdx.x - nmain) * L;
+    const int f0 = (int)blockIdx.x * L;
     const int nl = min(L, nface - f0);
-    for (int i = tid; i < nkz * L; i += WG) {
-        const int p = i % L, k = i / L;
-        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-        if (p < nl) {
-            const int f = f0 + p;
-            int xo, yo, xs, ys;
-            if (f < Ny) { xo = Nx - 1; yo = f; xs = 0; ys = f; }
-            else { xo = f - Ny; yo = Ny - 1; xs = f - Ny; ys = 0; }
-            a = spec[((long)k * Nx + xo) * Ny + yo];
-            b = spec[((long)k * Nx + xs) * Ny + ys];
-        }
-        if (k == 0 || k == N / 2) {
-            a.y = R(0);
-            b.y = R(0);
-        }
-        buf[k * LS + p] = {a.x - b.y, a.y + b.x};
-        if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+    if (tid < L) {
+        const int f = f0 + min(tid, nl - 1);
+        sqr[tid] = sq[f < Ny ? Nx - 1 : f - Ny];
+        sqr[L + tid] = sq[Nx + (f < Ny ? f : Ny - 1)];
     }
+    load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+        if (p >= nl) return 0;
+        const int f = f0 + p;
+        int xo, yo, xs, ys;
+        if (f < Ny) { xo = Nx - 1; yo = f; xs = 0; ys = f; }
+        else { xo = f - Ny; yo = Ny - 1; xs = f - Ny; ys = 0; }
+        ia = ((long)k * Nx + xo) * Ny + yo;
+        ib = ((long)k * Nx + xs) * Ny + ys;
+        return 3;
+    });
     __syncthreads();
     FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
-    for (int i = tid; i < L * N; i += WG) {
-        const int z = i % N, p = i / N;
-        if (p < nl) {
-            const int f = f0 + p;
-            int xo, yo;
-            bool from_src;
-            if (f < Ny) { xo = Nx - 1; yo = f; from_src = (f < Ny - 1) && (z < N - 1); }
-            else { xo = f - Ny; yo = Ny - 1; from_src = (z < N - 1); }
-            const cpx<R> c = buf[z * LS + p];
-            const double phi = (double)(from_src ? c.y : c.x);
-            const double pot = trap(tc, sq[xo], sq[Nx + yo], sq[Nx + Ny + z]);
-            const long n = ((long)xo * Ny + yo) * N + z;
-            psi[n] = nu_apply(psi[n], s, phi, pot, nc);
+    constexpr int QF = L * N / WG;
+    constexpr int CF = PREFETCH ? QF : (QF < 8 ? QF : 8);
+    static_assert((L * N) % WG == 0 && QF % CF == 0, "zinv_nu face tile");
+#pragma unroll
+    for (int q0 = 0; q0 < QF; q0 += CF) {
+        P fv[CF];
+        long fn[CF];
+#pragma unroll
+        for (int c = 0; c < CF; ++c) {
+            const int i = tid + (q0 + c) * WG;
+            const int z = i % N, f = f0 + min(i / N, nl - 1);
+            const int xo = f < Ny ? Nx - 1 : f - Ny, yo = f < Ny ? f : Ny - 1;
+            fn[c] = ((long)xo * Ny + yo) * N + z;
+            fv[c] = psi[fn[c]];
+        }
+#pragma unroll
+        for (int c = 0; c < CF; ++c) {
+            const int i = tid + (q0 + c) * WG;
+            const int z = i % N, p = i / N;
+            if (p < nl) {
+                const int f = f0 + p;
+                const bool from_src = (f < Ny) ? (f < Ny - 1) && (z < N - 1) : (z < N - 1);
+                const cpx<R> cv = buf[z * LS + p];
+                const double phi = (double)(from_src ? cv.y : cv.x);
+                const double pot = trap(tc, sqr[p], sqr[L + p], sqz[z]);
+                psi[fn[c]] = nu_apply(fv[c], s, phi, pot, nc);
+            }
         }
     }
 }
@@ -395,6 +490,8 @@ struct FftLaunch {
         static const int lm = fft_lm("DBEC_FFT_LM_ZFWD", 2);
         return lm == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
     }
+    // FP32 spectra use tiles twice as wide, so every strided access still
+    // moves 128-256 contiguous bytes.
 
     template <int LM>
     static hipError_t ypass_t(int dir, const FftArgs &a) {
@@ -414,7 +511,7 @@ struct FftLaunch {
         }
         return hipGetLastError();
     }
-    static hipError_t ypass(int dir, const FftArgs &a) { return ypass_t<1>(dir, a); }
+    static hipError_t ypass(int dir, const FftArgs &a) { return a.fp32 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a); }
 
     template <int LM>
     static hipError_t xconv_t(const FftArgs &a) {
@@ -431,7 +528,7 @@ struct FftLaunch {
                                                                 static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
     }
-    static hipError_t xconv(const FftArgs &a) { return xconv_t<1>(a); }
+    static hipError_t xconv(const FftArgs &a) { return a.fp32 ? xconv_t<2>(a) : xconv_t<1>(a); }
 
     static hipError_t zinv(const FftArgs &a) {
         using Tl = FftTile<N, 1>;
@@ -459,7 +556,7 @@ struct FftLaunch {
         const int nfaceb = (int)((a.Ny + a.Nx - 1 + Tl::L - 1) / Tl::L);
         dim3 grid((unsigned)(nmain + nfaceb)), block(Tl::WG);
         k_zinv_nu<N, R, P, LM><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx,
-                                                             (int)a.Ny, nmain, tw);
+                                                             (int)a.Ny, nfaceb, tw);
         return hipGetLastError();
     }
     template <int LM>
@@ -467,7 +564,7 @@ struct FftLaunch {
         if (!a.fp32) return a.cpsi ? fused_t<cpx<double>, double, LM>(a) : fused_t<double, double, LM>(a);
         return a.cpsi ? fused_t<cpx<float>, float, LM>(a) : fused_t<float, float, LM>(a);
     }
-    static hipError_t fused(const FftArgs &a) { return fused_lm<1>(a); }
+    static hipError_t fused(const FftArgs &a) { return a.fp32 ? fused_lm<2>(a) : fused_lm<1>(a); }
 };
 
 }  // namespace dbec

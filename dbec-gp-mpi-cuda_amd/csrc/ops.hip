@@ -135,23 +135,26 @@ hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale) {
 
 // ---------------------------------------------------------------------------
 // Thomas sweeps (calclux/calcluy/calcluz kernels, imag3d.cu:1802-1942;
-// real3d.cu:1902-2040).  A workgroup stages a tile of L lines in LDS with
-// coalesced loads (x and y lines: L consecutive z per row; z lines: whole
-// rows), one lane per line runs the reference recurrence on the LDS copy, and
-// the tile is written back.  cbeta lives in the LDS slots of psi: the
-// backward recurrence overwrites slot i-1 with cbeta[i-1] right after reading
-// psi[i-1], and the forward substitution overwrites slot i with the new
-// psi[i] after reading cbeta[i], so one LDS array per line suffices and no
-// intermediate touches HBM.  The LDS reads of each recurrence do not depend
-// on it, so they are issued in batches of U ahead of the dependent
-// arithmetic.  With COLS (z sweep, imaginary time) the lane also accumulates
-// the Simpson z integral of the new psi^2 and the wavefront sums the tile's
-// rows with the y weights: colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
+// real3d.cu:1902-2040), as a partitioned solve.  Every line is split into
+// P = n / M segments of M elements held in the registers of one thread.
+//
+// Backward (reference: cbeta[N-2] = psi[N-1]; cbeta[i-1] = cgamma[i] (A
+// cbeta[i] - c_i), c_i = minusA psi[i+1] + A0r psi[i] - A psi[i-1]): each
+// segment runs the same recurrence from a zero carry, Q, and the product of
+// its coefficients cgamma[m] A, Pb; then cbeta[i-1] = Q[i-1] + Pb[i-1] x with
+// x = cbeta at the segment's top, handed over from the segment on the right
+// (psi[N-1] for the last one).  Forward (psi'[0] = 0; psi'[i+1] = calpha[i]
+// psi'[i] + cbeta[i]; psi'[N-1] = 0): the same with the carry psi' coming
+// from the left.  Carries are folded through a small LDS exchange.  The
+// algebra is exact; results differ from the sequential sweep by round-off
+// only.  The coefficient tables sit in LDS so no step of the dependent
+// chains waits on a cache.  cbeta[i-1] replaces psi[i] in its register.
 __device__ __forceinline__ double m_mul(double a, double b) { return a * b; }
 __device__ __forceinline__ double m_add(double a, double b) { return a + b; }
 __device__ __forceinline__ double m_sub(double a, double b) { return a - b; }
 __device__ __forceinline__ double m_fma(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ double m_zero(double) { return 0.0; }
+__device__ __forceinline__ double m_one(double) { return 1.0; }
 __device__ __forceinline__ double m_mod2(double a) { return a * a; }
 __device__ __forceinline__ cpx<double> m_mul(cpx<double> a, cpx<double> b) { return cmul(a, b); }
 __device__ __forceinline__ cpx<double> m_add(cpx<double> a, cpx<double> b) { return a + b; }
@@ -164,6 +167,7 @@ __device__ __forceinline__ cpx<double> m_fma(cpx<double> a, cpx<double> b, cpx<d
     return {re, im};
 }
 __device__ __forceinline__ cpx<double> m_zero(cpx<double>) { return {0.0, 0.0}; }
+__device__ __forceinline__ cpx<double> m_one(cpx<double>) { return {1.0, 0.0}; }
 __device__ __forceinline__ double m_mod2(cpx<double> a) { return a.x * a.x + a.y * a.y; }
 
 // Storage <-> arithmetic conversions (FP32 storage computes in FP64 inside the line).
@@ -189,207 +193,62 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__device__ __forceinline__ double m_one(double) { return 1.0; }
-__device__ __forceinline__ cpx<double> m_one(cpx<double>) { return {1.0, 0.0}; }
+// LDS layout shared by both sweep kernels: coefficient tables (n each), the
+// carry exchange (one per thread), the last elements, the row partials.
+template <typename A>
+struct SweepShared {
+    A *ca, *cg, *ga, *exa, *exb, *exl;
+    double *exn;
+    __device__ SweepShared(char *raw, int n, int nthreads, int L) {
+        ca = reinterpret_cast<A *>(raw);
+        cg = ca + n;
+        ga = cg + n;
+        exa = ga + n;
+        exb = exa + nthreads;
+        exl = exb + nthreads;
+        exn = reinterpret_cast<double *>(exl + L);
+    }
+    __host__ __device__ static size_t bytes(int n, int nthreads, int L) {
+        return (3 * (size_t)n + 2 * (size_t)nthreads + L) * sizeof(A) + (size_t)nthreads * sizeof(double);
+    }
+};
 
-// Partitioned Thomas solve of L lines held in LDS (element e of line l at
-// buf[e * LS + l]) by 256 threads: thread (l, s) owns the steps
-// i in [ilo, ihi] of segment s of line l (P = 256 / L segments per line).
-//
-// Backward (reference: cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i), cbeta[N-2] =
-// psi[N-1]): each segment runs the same recurrence from a zero carry,
-// Q[i-1], and the product of its coefficients cgamma[m] A, P[i-1]; then
-// cbeta[i-1] = Q[i-1] + P[i-1] x with x = cbeta[ihi] handed over from the
-// segment on the right (psi[N-1] for the last one).  Forward (psi'[0] = 0,
-// psi'[i+1] = calpha[i] psi'[i] + cbeta[i]): the same with the carry psi'[ilo-1]
-// from the left.  The carries are folded through a small LDS exchange.  The
-// algebra is exact; the results differ from the sequential sweep only by
-// round-off.  cbeta[i-1] and the new psi'[i] are stored in slot i.
-template <typename A, int L, int LS, bool COLS>
-__device__ __forceinline__ void thomas_tile(A *__restrict__ buf, A *__restrict__ exa, A *__restrict__ exb,
-                                            double *__restrict__ exn, int n, int nl,
-                                            const A *__restrict__ calpha, const A *__restrict__ cgamma,
-                                            const SweepConst<A> &k, const double *__restrict__ wz,
-                                            const double *__restrict__ wy, int y0, double *colval) {
-    constexpr int P = 256 / L;
-    const int tid = threadIdx.x;
-    const int l = tid % L, s = tid / L;
-    const int m = n - 2;
-    const int ilo = 1 + (s * m) / P, ihi = ((s + 1) * m) / P;
-    A *f = buf + l;
-    const A zero = m_zero(k.A), one = m_one(k.A);
-    // halos: psi[ilo-1], psi[ihi], psi[ihi+1] before any slot is overwritten
-    A h_lo = zero, p_hi = zero, h_hi = zero;
-    if (ilo <= ihi) {
-        h_lo = f[(ilo - 1) * LS];
-        p_hi = f[ihi * LS];
-        h_hi = f[(ihi + 1) * LS];
+// The recursion coefficients are staged into LDS in two parts: the first
+// slot of each thread is loaded (coef_pre) before the thread's psi loads, so
+// writing it to LDS (coef_put) waits only for that older load while the psi
+// loads are still in flight; slots beyond the first thread count follow.
+template <typename A>
+struct CoefPre {
+    A ca, cg, ga;
+};
+template <typename A>
+__device__ __forceinline__ CoefPre<A> coef_pre(const A *ca, const A *cg, const A *ga, int n, int tid) {
+    const int i = tid < n - 1 ? tid : 0;
+    return {ca[i], cg[i], ga[i]};
+}
+template <typename A>
+__device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const A *ca, const A *cg,
+                                         const A *ga, int n, int tid, int nthreads) {
+    if (tid < n - 1) {
+        sh.ca[tid] = c.ca;
+        sh.cg[tid] = c.cg;
+        sh.ga[tid] = c.ga;
     }
-    __syncthreads();
-    // backward, zero carry
-    A bq = zero, bp = one;
-    {
-        A pn = h_hi, pc = p_hi;
-        for (int i = ihi; i >= ilo; --i) {
-            const A pp = (i - 1 >= ilo) ? f[(i - 1) * LS] : h_lo;
-            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-            const A g = cgamma[i];
-            bq = m_mul(g, m_sub(m_mul(k.A, bq), c));
-            bp = m_mul(m_mul(g, k.A), bp);
-            f[i * LS] = bq;
-            pn = pc;
-            pc = pp;
-        }
-    }
-    exa[s * L + l] = bq;
-    exb[s * L + l] = bp;
-    __syncthreads();
-    A x = f[(n - 1) * LS];  // cbeta[N-2] = psi[N-1]
-    for (int t = P - 1; t > s; --t) x = m_add(exa[t * L + l], m_mul(exb[t * L + l], x));
-    {
-        A pr = one;
-        for (int i = ihi; i >= ilo; --i) {
-            pr = m_mul(m_mul(cgamma[i], k.A), pr);
-            f[i * LS] = m_add(f[i * LS], m_mul(pr, x));
-        }
-    }
-    // forward, zero carry: slot i+1 holds cbeta[i] for i in [ilo-1, ihi-1]
-    A fy = zero, fr = one;
-    for (int i = ilo - 1; i <= ihi - 1; ++i) {
-        const A a = calpha[i];
-        fy = m_fma(a, fy, f[(i + 1) * LS]);
-        fr = m_mul(a, fr);
-        f[(i + 1) * LS] = fy;
-    }
-    __syncthreads();  // everyone has read its backward carries from exa/exb
-    exa[s * L + l] = fy;
-    exb[s * L + l] = fr;
-    __syncthreads();
-    A y = zero;  // psi'[0] = 0
-    for (int t = 0; t < s; ++t) y = m_add(exa[t * L + l], m_mul(exb[t * L + l], y));
-    double acc = 0.0;
-    {
-        A pr = one;
-        for (int i = ilo - 1; i <= ihi - 1; ++i) {
-            pr = m_mul(calpha[i], pr);
-            const A v = m_add(f[(i + 1) * LS], m_mul(pr, y));
-            f[(i + 1) * LS] = v;
-            if (COLS) acc += wz[i + 1] * m_mod2(v);
-        }
-    }
-    if (s == 0) f[0] = zero;
-    if (s == P - 1) f[(n - 1) * LS] = zero;
-    if constexpr (COLS) {
-        exn[s * L + l] = acc;
-        __syncthreads();
-        if (tid < 64) {
-            double w = 0.0;
-            if (tid < L && tid < nl) {
-                double row = 0.0;
-                for (int t = 0; t < P; ++t) row += exn[t * L + tid];
-                w = wy[y0 + tid] * row;
-            }
-            w = wave_sum(w);
-            if (tid == 0) *colval = w;
-        }
+    for (int i = tid + nthreads; i < n - 1; i += nthreads) {
+        sh.ca[i] = ca[i];
+        sh.cg[i] = cg[i];
+        sh.ga[i] = ga[i];
     }
 }
 
-template <typename S, int AXIS, int L, bool COLS>
-__global__ void __launch_bounds__(256)
-k_sweep(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
-        const typename Arith<S>::T *__restrict__ cgamma, SweepConst<typename Arith<S>::T> k,
-        double *__restrict__ colpart, const double *__restrict__ wz, const double *__restrict__ wy) {
-    using A = typename Arith<S>::T;
-    extern __shared__ char smem_raw[];
-    constexpr int LS = (AXIS == 2) ? L + 1 : L;
-    const int n = AXIS == 0 ? Nx : (AXIS == 1 ? Ny : Nz);
-    A *buf = reinterpret_cast<A *>(smem_raw);
-    A *exa = buf + n * LS;
-    A *exb = exa + 256;
-    double *exn = reinterpret_cast<double *>(exb + 256);
-    const int tid = threadIdx.x;
-    long base;
-    int nl, xrow = 0, y0 = 0, tile = 0;
-    if constexpr (AXIS == 0) {
-        const int zt = (Nz + L - 1) / L;
-        const int y = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
-        nl = min(L, Nz - z0);
-        base = (long)y * Nz + z0;
-    } else if constexpr (AXIS == 1) {
-        const int zt = (Nz + L - 1) / L;
-        const int x = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
-        nl = min(L, Nz - z0);
-        base = (long)x * Ny * Nz + z0;
-    } else {
-        const int yt = (Ny + L - 1) / L;
-        xrow = blockIdx.x / yt;
-        tile = blockIdx.x % yt;
-        y0 = tile * L;
-        nl = min(L, Ny - y0);
-        base = ((long)xrow * Ny + y0) * Nz;
-    }
-    const long estride = AXIS == 0 ? (long)Ny * Nz : (AXIS == 1 ? (long)Nz : 1L);
-    if constexpr (AXIS == 2) {
-        for (int i = tid; i < n * L; i += 256) {
-            const int e = i % n, l = i / n;
-            buf[e * LS + l] = (l < nl) ? to_a(psi[base + (long)l * Nz + e]) : m_zero(A{});
-        }
-    } else {
-        for (int i = tid; i < n * L; i += 256) {
-            const int l = i % L, e = i / L;
-            buf[e * LS + l] = (l < nl) ? to_a(psi[base + e * estride + l]) : m_zero(A{});
-        }
-    }
-    __syncthreads();
-    double colval = 0.0;
-    thomas_tile<A, L, LS, COLS>(buf, exa, exb, exn, n, nl, calpha, cgamma, k, wz, wy, y0, &colval);
-    if constexpr (COLS) {
-        if (tid == 0) colpart[(long)tile * Nx + xrow] = colval;
-    }
-    __syncthreads();
-    if constexpr (AXIS == 2) {
-        for (int i = tid; i < n * L; i += 256) {
-            const int e = i % n, l = i / n;
-            if (l < nl) psi[base + (long)l * Nz + e] = from_a<S>(buf[e * LS + l]);
-        }
-    } else {
-        for (int i = tid; i < n * L; i += 256) {
-            const int l = i % L, e = i / L;
-            if (l < nl) psi[base + e * estride + l] = from_a<S>(buf[e * LS + l]);
-        }
-    }
-}
-
-// x and y sweeps with register-resident segments.  A workgroup covers L
-// consecutive z (lanes, coalesced) of the lines of one (y) or (x) plane;
-// thread (l, s) holds the M elements [s*M, (s+1)*M) of line l in registers,
-// read straight from HBM, and runs the partitioned recurrence of thomas_tile
-// on them; only the segment carries go through LDS.  P = n / M segments.
-template <typename S, int AXIS, int L, int M>
-__global__ void __launch_bounds__(512)
-k_sweep_reg(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
-            const typename Arith<S>::T *__restrict__ cgamma, SweepConst<typename Arith<S>::T> k) {
-    using A = typename Arith<S>::T;
-    __shared__ A exa[512], exb[512], exf[512], exg[512], exl[64];
-    const int n = AXIS == 0 ? Nx : Ny;
-    const int P = n / M;
-    const int tid = threadIdx.x;
-    const int l = tid % L, s = tid / L;
-    const int zt = (Nz + L - 1) / L;
-    const int pl = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
-    const int nl = min(L, Nz - z0);
-    const long base = (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + l;
-    const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
-    const bool act = l < nl;
-    const int e0 = s * M;
+// The partitioned solve of one segment (registers v, halos hl / hr).  Every
+// thread of the workgroup calls it.  Returns, with COLS, the Simpson z
+// partial sum_j wz[e] |psi'|^2 of the segment.
+template <typename A, int M, bool COLS>
+__device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n, int s, int l, int L, int P,
+                                            const SweepShared<A> &sh, const SweepConst<A> &k,
+                                            const double *__restrict__ wz) {
     const A zero = m_zero(k.A), one = m_one(k.A);
-    A v[M];
-#pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? to_a(psi[base + (long)(e0 + j) * es]) : zero;
-    const A hl = (act && e0 > 0) ? to_a(psi[base + (long)(e0 - 1) * es]) : zero;
-    const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
-    // backward, zero carry; cbeta[i-1] replaces psi[i] in v[i - e0]
     A bq = zero, bp = one;
     {
         A pn = hr, pc = v[M - 1];
@@ -399,149 +258,302 @@ k_sweep_reg(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>
             const A pp = (j > 0) ? v[j - 1] : hl;
             if (i >= 1 && i <= n - 2) {
                 const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-                const A g = cgamma[i];
-                bq = m_mul(g, m_sub(m_mul(k.A, bq), c));
-                bp = m_mul(m_mul(g, k.A), bp);
+                bq = m_mul(sh.cg[i], m_sub(m_mul(k.A, bq), c));
+                bp = m_mul(sh.ga[i], bp);
                 v[j] = bq;
             }
             pn = pc;
             pc = pp;
         }
     }
-    exa[s * L + l] = bq;
-    exb[s * L + l] = bp;
-    if (s == P - 1) exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
+    sh.exa[s * L + l] = bq;
+    sh.exb[s * L + l] = bp;
+    if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
     __syncthreads();
-    A x = exl[l];
-    for (int t = P - 1; t > s; --t) x = m_add(exa[t * L + l], m_mul(exb[t * L + l], x));
+    A x = sh.exl[l];
+    for (int t = P - 1; t > s; --t) x = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], x));
     {
         A pr = one;
 #pragma unroll
         for (int j = M - 1; j >= 0; --j) {
             const int i = e0 + j;
             if (i >= 1 && i <= n - 2) {
-                pr = m_mul(m_mul(cgamma[i], k.A), pr);
+                pr = m_mul(sh.ga[i], pr);
                 v[j] = m_add(v[j], m_mul(pr, x));
             }
         }
     }
-    // forward, zero carry: v[j] holds cbeta[e0 + j - 1]
+    __syncthreads();  // carries read; the exchange is reused below
     A fy = zero, fr = one;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         const int i = e0 + j - 1;
         if (i + 1 >= 1 && i + 1 <= n - 2) {
-            const A a = calpha[i];
+            const A a = sh.ca[i];
             fy = m_fma(a, fy, v[j]);
             fr = m_mul(a, fr);
             v[j] = fy;
         }
     }
-    exf[s * L + l] = fy;
-    exg[s * L + l] = fr;
+    sh.exa[s * L + l] = fy;
+    sh.exb[s * L + l] = fr;
     __syncthreads();
-    A y = zero;
-    for (int t = 0; t < s; ++t) y = m_add(exf[t * L + l], m_mul(exg[t * L + l], y));
+    A y = zero;  // psi'[0] = 0
+    for (int t = 0; t < s; ++t) y = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], y));
+    double acc = 0.0;
     {
         A pr = one;
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             const int i = e0 + j - 1;
             if (i + 1 >= 1 && i + 1 <= n - 2) {
-                pr = m_mul(calpha[i], pr);
+                pr = m_mul(sh.ca[i], pr);
                 v[j] = m_add(v[j], m_mul(pr, y));
+                if (COLS) acc += wz[e0 + j] * m_mod2(v[j]);
             }
         }
     }
     if (s == 0) v[0] = zero;
     if (s == P - 1) v[M - 1] = zero;
+    return acc;
+}
+
+// x and y sweeps: a workgroup covers L consecutive z (lanes; one 256-byte row
+// per load) of the lines of one y (x sweep) or x (y sweep) plane; thread
+// (l, s) loads its segment straight from HBM into registers.
+template <typename S, int AXIS, int L, int M>
+__global__ void __launch_bounds__(512)
+k_sweep_xy(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
+           const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
+           SweepConst<typename Arith<S>::T> k) {
+    using A = typename Arith<S>::T;
+    extern __shared__ char smem_raw[];
+    const int n = AXIS == 0 ? Nx : Ny;
+    const int P = n / M;
+    const int nt = L * P;
+    SweepShared<A> sh(smem_raw, n, nt, L);
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int zt = (Nz + L - 1) / L;
+    const int pl = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
+    const int nl = min(L, Nz - z0);
+    const bool act = l < nl;
+    // lanes past the end of a partial tile read the last line (and discard it)
+    const long base = (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + (act ? l : nl - 1);
+    const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
+    const int e0 = s * M;
+    const A zero = m_zero(k.A);
+    const CoefPre<A> cp = coef_pre(calpha, cgamma, ga, n, tid);
+    // all loads in flight before any conversion
+    S raw[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) raw[j] = psi[base + (long)(e0 + j) * es];
+    const S rl = psi[base + (long)(e0 > 0 ? e0 - 1 : 0) * es];
+    const S rr = psi[base + (long)(e0 + M < n ? e0 + M : n - 1) * es];
+    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
+    A v[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) v[j] = act ? to_a(raw[j]) : zero;
+    const A hl = (act && e0 > 0) ? to_a(rl) : zero;
+    const A hr = (act && e0 + M < n) ? to_a(rr) : zero;
+    __syncthreads();
+    seg_solve<A, M, false>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
 #pragma unroll
         for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
     }
 }
 
-template <typename S, int AXIS>
-static hipError_t sweep_reg_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz,
-                                     const typename Arith<S>::T *ca, const typename Arith<S>::T *cg,
-                                     const SweepConst<typename Arith<S>::T> &k) {
+// z sweep: lines are contiguous rows, so a workgroup of L rows is staged
+// through an LDS tile in the storage type (row-major, one element of padding:
+// coalesced row loads and stores, conflict-free segment reads), each thread
+// moves its segment between the tile and its registers (FP64 arithmetic).
+// With COLS (imaginary time) the Simpson z integrals of the new psi^2 are
+// summed per row and the rows of the tile with the y weights:
+// colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
+template <typename S, int L, int M, bool COLS>
+__global__ void __launch_bounds__(512)
+k_sweep_z(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
+          const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
+          SweepConst<typename Arith<S>::T> k, double *__restrict__ colpart, const double *__restrict__ wz,
+          const double *__restrict__ wy) {
     using A = typename Arith<S>::T;
-    constexpr int L = sizeof(A) == 8 ? 32 : 16;  // 256-byte rows of z per load
-    const int n = AXIS == 0 ? Nx : Ny;
-    const int M = (sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 2) ? 32 : 16;
+    extern __shared__ char smem_raw[];
+    const int n = Nz;
+    const int RS = n + 1;  // row stride of the tile
     const int P = n / M;
-    if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
-    const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
-    dim3 grid((unsigned)nblocks), block(L * P);
-    if constexpr (sizeof(A) == 8) {
-        if (M == 32) {
-            k_sweep_reg<S, AXIS, L, 32><<<grid, block, 0, s>>>(psi, Nx, Ny, Nz, ca, cg, k);
-            return hipGetLastError();
+    const int nt = L * P;
+    SweepShared<A> sh(smem_raw, n, nt, L);
+    S *tile = reinterpret_cast<S *>(smem_raw + ((SweepShared<A>::bytes(n, nt, L) + 15) & ~size_t(15)));
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int yt = (Ny + L - 1) / L;
+    const int xrow = blockIdx.x / yt, tno = blockIdx.x % yt, y0 = tno * L;
+    const int nl = min(L, Ny - y0);
+    const long base = ((long)xrow * Ny + y0) * Nz;
+    const A zero = m_zero(k.A);
+    // The tile's nl rows are one contiguous run of nl * n elements; each
+    // thread moves M / VE packets of it (all loads first).
+    constexpr int VE = Pk<S>::n, KV = M / VE;
+    static_assert(M % VE == 0, "z sweep packets");
+    const int npk = nl * n / VE;
+    const CoefPre<A> cp = coef_pre(calpha, cgamma, ga, n, tid);
+    typename Pk<S>::type r[KV];
+#pragma unroll
+    for (int q = 0; q < KV; ++q) {
+        const int iv = tid + q * nt;
+        r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
+    }
+    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
+#pragma unroll
+    for (int q = 0; q < KV; ++q) {
+        const int iv = tid + q * nt;
+        if (iv < npk) {
+            const int e = (iv * VE) % n, row = (iv * VE) / n;
+#pragma unroll
+            for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
         }
     }
-    k_sweep_reg<S, AXIS, L, 16><<<grid, block, 0, s>>>(psi, Nx, Ny, Nz, ca, cg, k);
-    return hipGetLastError();
+    __syncthreads();
+    const int e0 = s * M;
+    const bool act = l < nl;
+    const S *row = tile + l * RS;
+    A v[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) v[j] = act ? to_a(row[e0 + j]) : zero;
+    const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
+    const A hr = (act && e0 + M < n) ? to_a(row[e0 + M]) : zero;
+    const double acc = seg_solve<A, M, COLS>(v, hl, hr, e0, n, s, l, L, P, sh, k, wz);
+    // every thread read its segment before the first barrier inside seg_solve
+    if (act) {
+        S *orow = tile + l * RS;
+#pragma unroll
+        for (int j = 0; j < M; ++j) orow[e0 + j] = from_a<S>(v[j]);
+    }
+    if constexpr (COLS) {
+        sh.exn[s * L + l] = acc;
+        __syncthreads();
+        if (tid < 64) {
+            double w = 0.0;
+            if (tid < L && tid < nl) {
+                double rsum = 0.0;
+                for (int t = 0; t < P; ++t) rsum += sh.exn[t * L + tid];
+                w = wy[y0 + tid] * rsum;
+            }
+            w = wave_sum(w);
+            if (tid == 0) colpart[(long)tno * Nx + xrow] = w;
+        }
+    } else {
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < KV; ++q) {
+        const int iv = tid + q * nt;
+        if (iv < npk) {
+            const int e = (iv * VE) % n, row = (iv * VE) / n;
+            typename Pk<S>::type o;
+#pragma unroll
+            for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
+            st_pk(psi + base + (long)iv * VE, o);
+        }
+    }
 }
 
-// LDS bytes of a sweep workgroup: the tile plus the carry exchange.
-static size_t sweep_lds(long n, int L, size_t elem) {
-    return (size_t)n * (L + 1) * elem + 2 * 256 * elem + 256 * sizeof(double);
+// Segment length of a line of n elements: 32 for real FP64 lines that split
+// into at least 4 segments, else 16 (fewer registers for complex lines).
+template <typename A>
+static int seg_len(long n) {
+    if (sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 4) return 32;
+    return 16;
 }
 
-// Lines per tile: the largest power of two <= 64 whose LDS fits the budget
-// (so several workgroups share a CU while others load or store), but with
-// segments of at least two steps.  DBEC_SWEEP_LDS_KB overrides the budget.
-static int sweep_lines(long n, size_t elem) {
-    static long budget = [] {
-        const char *e = getenv("DBEC_SWEEP_LDS_KB");
-        long kb = e ? atol(e) : 48;
-        return (kb > 8 && kb <= 160 ? kb : 48) * 1024;
-    }();
+// z sweep: 16-element segments (more threads per tile row); rows per
+// workgroup: a power of two >= 4 whose tile fits 48 KiB of LDS, with at most
+// 512 threads.
+static int z_seg_len(long) { return 16; }
+
+template <typename S>
+static int z_rows(long n) {
+    const int M = z_seg_len(n);
+    const int P = (int)(n / M);
     int L = 64;
-    while (L > 4 && (long)sweep_lds(n, L, elem) > budget) L /= 2;
-    while (L < 64 && 256 / L > (n - 2) / 2) L *= 2;
+    while (L > 4 && ((long)L * P > 512 || (long)(n + 1) * L * (long)sizeof(S) > 49152)) L /= 2;
     return L;
 }
 
 static size_t arith_elem(const dbec_ctx *c) { return c->mode == DBEC_REAL ? 16 : 8; }
 
 long sweep_z_tiles(const dbec_ctx *c) {
-    const int L = sweep_lines(c->Nz, arith_elem(c));
+    const bool f32 = c->prec == DBEC_MIXED;
+    int L;
+    if (c->mode == DBEC_IMAG) L = f32 ? z_rows<float>(c->Nz) : z_rows<double>(c->Nz);
+    else L = f32 ? z_rows<cpx<float>>(c->Nz) : z_rows<cpx<double>>(c->Nz);
     return (c->Ny + L - 1) / L;
 }
 
-template <typename S, int AXIS, bool COLS>
-static hipError_t sweep_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const void *ca, const void *cg,
+template <typename S, int AXIS>
+static hipError_t sweep_xy_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                    const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+                                    const SweepConst<typename Arith<S>::T> &k) {
+    using A = typename Arith<S>::T;
+    constexpr int L = 256 / sizeof(S);  // 256-byte rows of z per load
+    const int n = AXIS == 0 ? Nx : Ny;
+    int M = seg_len<A>(n);
+    if (L * (n / M) > 512) M = 32;
+    const int P = n / M;
+    if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
+    const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
+    dim3 grid((unsigned)nblocks), block(L * P);
+    const size_t lds = SweepShared<A>::bytes(n, L * P, L);
+    if constexpr (sizeof(A) == 8) {
+        if (M == 32) {
+            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k);
+            return hipGetLastError();
+        }
+    }
+    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k);
+    return hipGetLastError();
+}
+
+template <typename S, int L, bool COLS>
+static hipError_t sweep_z_launch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                 const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
                                  const SweepConst<typename Arith<S>::T> &k, double *colpart, const double *wz,
                                  const double *wy) {
     using A = typename Arith<S>::T;
-    const int n = AXIS == 0 ? Nx : (AXIS == 1 ? Ny : Nz);
-    const int other = AXIS == 2 ? Ny : Nz;
-    const int L = sweep_lines(n, sizeof(A));
-    const int tiles = (other + L - 1) / L;
-    const long nblocks = (long)tiles * (AXIS == 0 ? Ny : Nx);
-    const size_t lds = sweep_lds(n, L, sizeof(A));
-    dim3 grid((unsigned)nblocks), block(256);
-    const A *cal = static_cast<const A *>(ca), *cgm = static_cast<const A *>(cg);
-    switch (L) {
-    case 64: k_sweep<S, AXIS, 64, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
-    case 32: k_sweep<S, AXIS, 32, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
-    case 16: k_sweep<S, AXIS, 16, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
-    case 8: k_sweep<S, AXIS, 8, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
-    default: k_sweep<S, AXIS, 4, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, cal, cgm, k, colpart, wz, wy); break;
-    }
+    const int P = Nz / 16;
+    const long nblocks = (long)((Ny + L - 1) / L) * Nx;
+    dim3 grid((unsigned)nblocks), block(L * P);
+    const size_t lds = ((SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15)) + (size_t)(Nz + 1) * L * sizeof(S);
+    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
     return hipGetLastError();
+}
+
+template <typename S, bool COLS>
+static hipError_t sweep_z_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                   const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+                                   const SweepConst<typename Arith<S>::T> &k, double *colpart, const double *wz,
+                                   const double *wy) {
+    using A = typename Arith<S>::T;
+    if (Nz % 16 != 0) return hipErrorInvalidValue;
+    switch (z_rows<S>(Nz)) {
+    case 64: return sweep_z_launch<S, 64, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    case 32: return sweep_z_launch<S, 32, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    case 16: return sweep_z_launch<S, 16, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    case 8: return sweep_z_launch<S, 8, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    default: return sweep_z_launch<S, 4, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    }
 }
 
 template <typename S, typename A>
 static hipError_t sweep_axis(hipStream_t s, S *psi, const dbec_ctx *c, int axis, const A *ca, const A *cg,
-                             const SweepConst<A> &k, double *colpart) {
+                             const A *ga, const SweepConst<A> &k, double *colpart) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
-    if (axis == 0) return sweep_reg_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, k);
-    if (axis == 1) return sweep_reg_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, k);
-    if (colpart) return sweep_dispatch<S, 2, true>(s, psi, Nx, Ny, Nz, ca, cg, k, colpart, wz, wy);
-    return sweep_dispatch<S, 2, false>(s, psi, Nx, Ny, Nz, ca, cg, k, nullptr, wz, wy);
+    if (axis == 0) return sweep_xy_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, ga, k);
+    if (axis == 1) return sweep_xy_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, ga, k);
+    if (colpart) return sweep_z_dispatch<S, true>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    return sweep_z_dispatch<S, false>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, nullptr, wz, wy);
 }
 
 hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colpart) {
@@ -552,22 +564,24 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colp
     if (c->mode == DBEC_IMAG) {
         const double *ca = b.coef + off[axis];
         const double *cg = b.coef + tot + off[axis];
+        const double *ga = b.coef + 2 * tot + off[axis];
         SweepConst<double> k;
         k.A0r = c->A0r[axis][0];
         k.A = c->A[axis][0];
         k.mA = -c->A[axis][0];
-        return f32 ? sweep_axis(s, static_cast<float *>(b.psi), c, axis, ca, cg, k, colpart)
-                   : sweep_axis(s, static_cast<double *>(b.psi), c, axis, ca, cg, k, colpart);
+        return f32 ? sweep_axis(s, static_cast<float *>(b.psi), c, axis, ca, cg, ga, k, colpart)
+                   : sweep_axis(s, static_cast<double *>(b.psi), c, axis, ca, cg, ga, k, colpart);
     }
     const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
     const cpx<double> *ca = cc + off[axis];
     const cpx<double> *cg = cc + tot + off[axis];
+    const cpx<double> *ga = cc + 2 * tot + off[axis];
     SweepConst<cpx<double>> k;
     k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
     k.A = {c->A[axis][0], c->A[axis][1]};
     k.mA = {c->mA[axis][0], c->mA[axis][1]};
-    return f32 ? sweep_axis(s, static_cast<cpx<float> *>(b.psi), c, axis, ca, cg, k, colpart)
-               : sweep_axis(s, static_cast<cpx<double> *>(b.psi), c, axis, ca, cg, k, colpart);
+    return f32 ? sweep_axis(s, static_cast<cpx<float> *>(b.psi), c, axis, ca, cg, ga, k, colpart)
+               : sweep_axis(s, static_cast<cpx<double> *>(b.psi), c, axis, ca, cg, ga, k, colpart);
 }
 
 // norm^2 = sum_x wx[x] sum_t colpart[t][x]; scal[NORM] = norm, scal[SCALE] = 1/norm.

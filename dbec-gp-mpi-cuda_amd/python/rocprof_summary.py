@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Summaries of rocprofv3 CSV output for the solver's kernels.
+
+    rocprof_summary.py stats  <run_kernel_stats.csv>                 [--title T] [--cmd C]
+    rocprof_summary.py traffic <fetch counter_collection.csv> <write counter_collection.csv>
+                               --workload W [--copy-mib M] [--cmd C]
+
+`stats` prints a Markdown table of per-kernel calls and durations (from
+`rocprofv3 --kernel-trace --stats --output-format csv`).  `traffic` turns two
+`rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs into HBM bytes per
+launch for each kernel, JSON on stdout.  Both counters are in KiB per
+dispatch; each is scaled by what it reads for the runtime's device-to-device
+copy kernel (`__amd_rocclr_copyBuffer`) of a known size M MiB that the
+profiled program ran first (bench.py --copy-calibration-mb M): a copy of M MiB
+fetches and writes M MiB, so factor = M MiB / sum of the counter over the copy
+dispatches.
+"""
+import argparse
+import csv
+import json
+import re
+import sys
+from collections import defaultdict
+
+
+def template_args(full, start):
+    """Top-level template arguments of the name whose '<' is at `start`."""
+    depth, cur, out = 0, "", []
+    for ch in full[start:]:
+        if ch == "<":
+            depth += 1
+            if depth == 1:
+                continue
+        elif ch == ">":
+            depth -= 1
+            if depth == 0:
+                out.append(cur.strip())
+                return out
+        elif ch == "," and depth == 1:
+            out.append(cur.strip())
+            cur = ""
+            continue
+        cur += ch
+    return out
+
+
+def short_name(full):
+    """Kernel name as bench.py reports it, or None for kernels outside the step."""
+    m = re.search(r"dbec::(k_\w+)", full)
+    if not m:
+        return "copy" if "copyBuffer" in full else None
+    k = m.group(1)
+    targs = template_args(full, m.end()) if full[m.end():m.end() + 1] == "<" else []
+    if k == "k_ypass":
+        return "ypass_fwd" if targs[1] == "-1" else "ypass_inv"
+    if k == "k_sweep_xy":  # template <typename S, int AXIS, int L, int M>
+        return "sweep_x" if targs[1] == "0" else "sweep_y"
+    return {
+        "k_zfwd": "zfwd", "k_xconv": "xconv", "k_zinv_nu": "zinv_nu", "k_zinv": "zinv",
+        "k_sweep_z": "sweep_z", "k_norm_cols": "norm", "k_scale": "scale",
+    }.get(k, k[2:])
+
+
+def cmd_stats(args):
+    rows = list(csv.DictReader(open(args.csv)))
+    agg = defaultdict(lambda: [0, 0.0, None])
+    for r in rows:
+        n = short_name(r["Name"]) or r["Name"][:40]
+        a = agg[n]
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"])
+        a[2] = r["Name"]
+    total = sum(a[1] for a in agg.values())
+    print(f"# {args.title}\n")
+    if args.cmd:
+        print(f"Command: `{args.cmd}`\n")
+    print("| kernel | calls | avg us | share | full name |")
+    print("|---|---:|---:|---:|---|")
+    for n, (calls, tot, full) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {n} | {calls} | {tot / calls / 1e3:.1f} | {100 * tot / total:.1f}% | `{full[:110]}` |")
+
+
+def counter_avg(path, counter):
+    per = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        per[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return per
+
+
+def cmd_traffic(args):
+    fetch = counter_avg(args.fetch_csv, "FETCH_SIZE")
+    write = counter_avg(args.write_csv, "WRITE_SIZE")
+    mib = args.copy_mib
+
+    def factor(per):
+        vals = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
+        if not vals:
+            raise SystemExit("no copy dispatch found: run the program with --copy-calibration-mb")
+        return mib * 1024.0 / sum(vals)
+
+    ff, fw = factor(fetch), factor(write)
+    out = {"workload": args.workload, "command": args.cmd, "copy_calibration_mib": mib,
+           "fetch_factor": ff, "write_factor": fw, "kernels": {}}
+    by_short = defaultdict(lambda: {"f": [], "w": []})
+    for k, vs in fetch.items():
+        s = short_name(k)
+        if s and s != "copy":
+            by_short[s]["f"] += vs
+    for k, vs in write.items():
+        s = short_name(k)
+        if s and s != "copy":
+            by_short[s]["w"] += vs
+    for s, d in sorted(by_short.items()):
+        if not d["f"] or not d["w"]:
+            continue
+        out["kernels"][s] = {
+            "dispatches": len(d["f"]),
+            "fetch_kib_raw": sum(d["f"]) / len(d["f"]),
+            "write_kib_raw": sum(d["w"]) / len(d["w"]),
+            "fetch_bytes": ff * 1024.0 * sum(d["f"]) / len(d["f"]),
+            "write_bytes": fw * 1024.0 * sum(d["w"]) / len(d["w"]),
+        }
+    json.dump(out, sys.stdout, indent=1)
+    print()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("--title", default="rocprofv3 kernel statistics")
+    a.add_argument("--cmd", default="")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_csv")
+    b.add_argument("write_csv")
+    b.add_argument("--workload", required=True)
+    b.add_argument("--copy-mib", type=float, default=512)
+    b.add_argument("--cmd", default="")
+    args = ap.parse_args()
+    cmd_stats(args) if args.what == "stats" else cmd_traffic(args)
+
+
+if __name__ == "__main__":
+    main()
