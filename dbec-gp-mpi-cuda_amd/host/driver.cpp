@@ -12,11 +12,14 @@
  * Extension keys (absent by default, so default runs behave like the
  * reference):
  *   PRECISION = FP64 | MIXED      arithmetic of the bulk operators
- *   POLISH    = K                 after a MIXED imaginary-time run, K more
- *                                 steps in FP64 (printed as a separate line)
- *   OBSOUT    = name              write name.txt with norm, mu, E, rms every
- *   OBSSTP    = K                 K steps of NPAS and NRUN (outside the
- *                                 timed region; does not change psi)
+ *   POLISH    = K                 after a MIXED imaginary-time NRUN, K more
+ *                                 steps in FP64 before the NRUN report
+ *   OBSSTP    = K                 observables every K steps of NPAS and NRUN
+ *                                 (outside the timed region; psi unchanged)
+ *   OBSOUT    = name              write them to name.txt (needs OBSSTP)
+ *   MUTOL     = eps               imaginary time: end NPAS / NRUN early once
+ *                                 |mu - mu_prev| <= eps |mu| between two
+ *                                 OBSSTP points (needs OBSSTP)
  */
 #include "dbec.h"
 
@@ -79,6 +82,7 @@ struct Params {
     long polish = 0;
     const char *obsout = nullptr;
     long obsstp = 0;
+    double mutol = 0.;
 };
 
 [[noreturn]] void die(const char *msg, bool to_stdout = false) {
@@ -181,9 +185,19 @@ void readpar(const Cfg &cfg, Params &P, bool real) {
         else if (!std::strcmp(t, "MIXED")) P.prec = DBEC_MIXED;
         else die("PRECISION can be FP64 or MIXED.\n");
     }
-    if ((t = cfg.read("POLISH")) != nullptr) P.polish = std::atol(t);
+    if ((t = cfg.read("POLISH")) != nullptr) {
+        P.polish = std::atol(t);
+        if (P.polish < 0) die("POLISH must not be negative.\n");
+        if (P.polish > 0 && (real || P.prec != DBEC_MIXED))
+            die("POLISH applies to an imaginary-time run with PRECISION = MIXED.\n");
+    }
     P.obsout = cfg.read("OBSOUT");
-    if (P.obsout) {
+    if ((t = cfg.read("MUTOL")) != nullptr) {
+        P.mutol = std::atof(t);
+        if (real) die("MUTOL applies to imaginary time only.\n");
+        if (!(P.mutol > 0.)) die("MUTOL must be positive.\n");
+    }
+    if (P.obsout || P.mutol > 0.) {
         if ((t = cfg.read("OBSSTP")) == nullptr) die("OBSSTP is not defined in the configuration file.\n");
         P.obsstp = std::atol(t);
         if (P.obsstp < 1) die("OBSSTP must be positive.\n");
@@ -317,13 +331,17 @@ struct ObsLog {
         std::fprintf(f, "# %s: step  t  norm  mu/par  E/par  <r>  <x>  <y>  <z>\n",
                      real ? "real time" : "imaginary time");
     }
-    void line(dbec_ctx *ctx, long step, double t, double par) {
-        if (!f) return;
+    /* Measures the observables (norm measured, no rescale) and logs them;
+     * returns mu. */
+    double line(dbec_ctx *ctx, long step, double t, double par) {
         Obs o;
         observe(ctx, o, true);
-        std::fprintf(f, "%ld %.10e %.16e %.16e %.16e %.16e %.16e %.16e %.16e\n", step, t, o.norm,
-                     o.mu / par, o.en / par, o.rms[0], o.rms[1], o.rms[2], o.rms[3]);
-        std::fflush(f);
+        if (f) {
+            std::fprintf(f, "%ld %.10e %.16e %.16e %.16e %.16e %.16e %.16e %.16e\n", step, t, o.norm,
+                         o.mu / par, o.en / par, o.rms[0], o.rms[1], o.rms[2], o.rms[3]);
+            std::fflush(f);
+        }
+        return o.mu;
     }
     ~ObsLog() {
         if (f) std::fclose(f);
@@ -492,11 +510,14 @@ int run(int argc, char **argv, bool real) {
     /* A phase of `n` steps with observables every OBSSTP steps (extension) and,
      * in real time, DYNAOUT rms every OUTSTPT steps inside the timed loop
      * (real3d.cu:419-423, 504-507). */
-    auto phase = [&](long n, long label_base, bool dyn) {
+    const bool cadence = P.obsstp > 0;
+    auto phase = [&](long n, long label_base, bool dyn, const char *name) {
         long done = 0;
+        double mu_prev = 0.;
+        bool have_prev = false;
         while (done < n) {
             long chunk = n - done;
-            if (olog.f) chunk = std::min(chunk, P.obsstp - (done % P.obsstp));
+            if (cadence) chunk = std::min(chunk, P.obsstp - (done % P.obsstp));
             if (dyn) {
                 /* next step count d with (d - 1) % OUTSTPT == 0 */
                 long nxt = ((done + P.outstpt - 1) / P.outstpt) * P.outstpt + 1;
@@ -514,8 +535,19 @@ int run(int argc, char **argv, bool real) {
                 std::fprintf(dyna, "%8le   %8le   %8le   %8le   %8le\n", tl, rms[0], rms[1], rms[2], rms[3]);
                 std::fflush(dyna);
             }
-            if (olog.f && done % P.obsstp == 0) olog.line(ctx, done + label_base, (done + label_base) * P.dt * par, par);
+            if (cadence && done % P.obsstp == 0) {
+                const double mu = olog.line(ctx, done + label_base, (done + label_base) * P.dt * par, par);
+                if (P.mutol > 0. && have_prev && std::fabs(mu - mu_prev) <= P.mutol * std::fabs(mu)) {
+                    std::fprintf(out, " * %s converged after %ld steps: |dmu/mu| = %.3e *\n", name, done,
+                                 std::fabs(mu - mu_prev) / std::fabs(mu));
+                    std::fflush(out);
+                    break;
+                }
+                mu_prev = mu;
+                have_prev = true;
+            }
         }
+        return done;
     };
 
     const char *phase_line[3] = {"After NSTP iter.:", "After NPAS iter.:", "After NRUN iter.:"};
@@ -562,7 +594,7 @@ int run(int argc, char **argv, bool real) {
     }
 
     /* NPAS. */
-    phase(P.Npas, 0, real && dyna != nullptr);
+    const long npas_done = phase(P.Npas, 0, real && dyna != nullptr, "NPAS");
     report(1, P.Npasout);
 
     /* real3d quench before NRUN (real3d.cu:493-494). */
@@ -572,7 +604,27 @@ int run(int argc, char **argv, bool real) {
     }
 
     /* NRUN. */
-    phase(P.Nrun, P.Npas, real && dyna != nullptr);
+    /* MUTOL ends NPAS early in imaginary time only, so real time labels
+     * with the full NPAS as the reference does */
+    phase(P.Nrun, npas_done, real && dyna != nullptr, "NRUN");
+
+    /* Extension: FP64 polish of a mixed-precision ground state.  psi moves
+     * to an FP64 solver with the same grid and couplings, which takes the
+     * remaining observables and outputs. */
+    if (P.polish > 0) {
+        check(dbec_get_coupling(ctx, &g, &gd), ctx);
+        std::vector<double> host(N);
+        check(dbec_get_psi(ctx, host.data()), ctx);
+        dbec_ctx *fine = nullptr;
+        check(dbec_create(&dp, DBEC_IMAG, DBEC_FP64, nullptr, &fine), nullptr);
+        check(dbec_destroy(ctx), nullptr);
+        ctx = fine;
+        check(dbec_set_psi(ctx, host.data()), ctx);
+        check(dbec_set_coupling(ctx, g, gd), ctx);
+        steps(P.polish, 0., 0.);
+        std::fprintf(out, " * POLISH: %ld FP64 steps *\n", P.polish);
+        std::fflush(out);
+    }
     report(2, P.Nrunout);
 
     if (filerms) std::fclose(filerms);

@@ -140,3 +140,75 @@ def test_real3d_without_imag3d_den(bindir, tmp_path):
     r = run(bindir, "real3d", ["-i", "in"], tmp_path)
     assert r.returncode == 1
     assert r.stdout.endswith("Run the program using the input file to read. e.g.: imag3d-den.bin\n")
+
+
+# ---- extension keys (absent from the reference; defaults leave its behaviour)
+
+@pytest.mark.parametrize("extra,msg", [
+    ({"MUTOL": "1e-8"}, "OBSSTP is not defined in the configuration file.\n"),
+    ({"OBSOUT": "obs"}, "OBSSTP is not defined in the configuration file.\n"),
+    ({"OBSOUT": "obs", "OBSSTP": "0"}, "OBSSTP must be positive.\n"),
+    ({"MUTOL": "-1", "OBSSTP": "5"}, "MUTOL must be positive.\n"),
+    ({"PRECISION": "FP16"}, "PRECISION can be FP64 or MIXED.\n"),
+    ({"POLISH": "10"}, "POLISH applies to an imaginary-time run with PRECISION = MIXED.\n"),
+])
+def test_extension_key_errors(bindir, extra, msg, tmp_path):
+    write(tmp_path / "in", dict(BASE, **extra))
+    r = run(bindir, "imag3d", ["-i", "in"], tmp_path)
+    assert r.returncode == 1 and r.stderr == msg
+
+
+def test_mutol_real_time_rejected(bindir, tmp_path):
+    write(tmp_path / "in", dict(BASE, GPAR="1.", GDDPAR="1.", MUTOL="1e-8", OBSSTP="5"))
+    r = run(bindir, "real3d", ["-i", "in"], tmp_path)
+    assert r.returncode == 1 and r.stderr == "MUTOL applies to imaginary time only.\n"
+
+
+CONV = dict(BASE, G0="20.", GDD0="10.", CUTOFF="4.", DX="0.6", DY="0.6", DZ="0.6", DT="0.01", NPAS="3000",
+            NRUN="0", OUTPUT="out")
+
+
+def phase_lines(path):
+    return [l for l in open(path).read().splitlines() if l.startswith(("After", " *"))]
+
+
+def test_mutol_stops_npas_where_a_fixed_count_ends(oracle_bin, tmp_path):
+    """MUTOL ends NPAS at the first OBSSTP point with |dmu/mu| <= MUTOL; the
+    state is the one a run of exactly that many NPAS steps reaches (the
+    observables at the cadence leave psi unchanged)."""
+    a = tmp_path / "a"
+    a.mkdir()
+    write(a / "in", dict(CONV, OBSSTP="50", OBSOUT="obs", MUTOL="1e-7"))
+    r = run(oracle_bin, "imag3d", ["-i", "in"], a)
+    assert r.returncode == 0, r.stderr
+    lines = phase_lines(a / "out.txt")
+    stop = [l for l in lines if "NPAS converged after" in l]
+    assert len(stop) == 1, lines
+    n = int(stop[0].split("after")[1].split()[0])
+    assert 0 < n < 3000 and n % 50 == 0
+    rows = [list(map(float, l.split())) for l in open(a / "obs.txt") if not l.startswith("#")]
+    assert int(rows[-1][0]) == n
+    mu1, mu0 = rows[-1][3], rows[-2][3]
+    assert abs(mu1 - mu0) <= 1e-7 * abs(mu1)
+    assert all(abs(r1[3] - r0[3]) > 1e-7 * abs(r1[3]) for r0, r1 in zip(rows[:-2], rows[1:-1]))
+    b = tmp_path / "b"
+    b.mkdir()
+    write(b / "in", dict(CONV, NPAS=str(n)))
+    r = run(oracle_bin, "imag3d", ["-i", "in"], b)
+    assert r.returncode == 0, r.stderr
+    assert [l for l in lines if l.startswith("After")] == phase_lines(b / "out.txt")
+
+
+def test_obs_cadence_leaves_results_unchanged(oracle_bin, tmp_path):
+    out = []
+    for name, extra in (("plain", {}), ("obs", {"OBSSTP": "7", "OBSOUT": "obs"})):
+        d = tmp_path / name
+        d.mkdir()
+        write(d / "in", dict(CONV, NPAS="40", NRUN="9", NRUNOUT="den", OUTSTPX="1", OUTSTPY="1", OUTSTPZ="1",
+                             **extra))
+        r = run(oracle_bin, "imag3d", ["-i", "in"], d)
+        assert r.returncode == 0, r.stderr
+        out.append((phase_lines(d / "out.txt"), (d / "den1d_x.bin").read_bytes()))
+    assert out[0] == out[1]
+    rows = [l for l in open(tmp_path / "obs" / "obs.txt") if not l.startswith("#")]
+    assert [int(l.split()[0]) for l in rows] == [7, 14, 21, 28, 35, 47]
