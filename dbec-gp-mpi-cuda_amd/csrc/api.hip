@@ -94,6 +94,12 @@ int hip_fail(dbec_ctx *c, hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail((c), e_, (what)); \
     } while (0)
 
+#define CHK(expr)                    \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_ != DBEC_OK) return rc_; \
+    } while (0)
+
 size_t psi_elem(const dbec_ctx *c) {
     size_t r = (c->prec == DBEC_MIXED) ? sizeof(float) : sizeof(double);
     return c->mode == DBEC_REAL ? 2 * r : r;
@@ -116,7 +122,26 @@ hipError_t timed(dbec_ctx *c, const char *name, double bytes, F &&launch) {
     return e;
 }
 
-FftArgs fft_args(const dbec_ctx *c, int axis, const double *scale) {
+// The solve held by a single context; use_scale: psi is read as psi / norm of
+// the last step (renormalisation pending).
+Solves solve_ref(const dbec_ctx *c, bool use_scale) {
+    Solves v;
+    v.psi = c->b.psi;
+    v.spec = c->b.spec;
+    v.rows = c->b.rows;
+    v.scal = c->b.scal;
+    v.psi_st = v.spec_st = v.rows_st = v.scal_st = 0;
+    v.slot = nullptr;
+    v.par = nullptr;
+    v.p0.tc = trap_const(c);
+    v.p0.g = c->g;
+    v.p0.gd = c->gd;
+    v.nb = 1;
+    v.use_scale = use_scale ? 1 : 0;
+    return v;
+}
+
+FftArgs fft_args(const dbec_ctx *c, int axis, const Solves &v) {
     FftArgs a;
     a.stream = c->stream;
     a.Nx = c->Nx;
@@ -125,29 +150,28 @@ FftArgs fft_args(const dbec_ctx *c, int axis, const double *scale) {
     a.Nkz = c->Nkz;
     a.fp32 = c->prec == DBEC_MIXED;
     a.cpsi = c->mode == DBEC_REAL;
-    a.psi = c->b.psi;
-    a.scale = scale;
-    a.spec = c->b.spec;
+    a.v = v;
     a.phi = c->b.phi;
     a.potddq = c->b.potddq;
     a.tw = a.fp32 ? c->b.twf[axis] : c->b.tw[axis];
     a.sq = c->b.sq;
-    a.tc = trap_const(c);
     a.nc = nu_const(c);
     return a;
 }
 
-// calcpsidd2: |psi|^2 -> half spectrum -> * potdd -> phi, then the face copies.
-int run_calcpsidd2(dbec_ctx *c, const double *scale) {
+// calcpsidd2 of the context's own psi (no pending scale): |psi|^2 -> half
+// spectrum -> * potdd -> phi, then the face copies.
+int run_calcpsidd2(dbec_ctx *c) {
+    const Solves v = solve_ref(c, false);
     const double ce = (double)(2 * real_elem(c));
     const double spec_b = (double)c->Nkz * c->Nx * c->Ny * ce;
     const double nre = (double)c->N * real_elem(c);
-    HIPCHK(c, timed(c, "zfwd", (double)c->N * psi_elem(c) + spec_b, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, scale)); }), "zfwd");
-    HIPCHK(c, timed(c, "ypass_fwd", 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, nullptr)); }), "ypass");
+    HIPCHK(c, timed(c, "zfwd", (double)c->N * psi_elem(c) + spec_b, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }), "zfwd");
+    HIPCHK(c, timed(c, "ypass_fwd", 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }), "ypass");
     const double pq = (double)c->Nkz * (c->Nx / 2 + 1) * (c->Ny / 2 + 1) * real_elem(c);
-    HIPCHK(c, timed(c, "xconv", 2 * spec_b + pq, [&] { return launch_xconv(c->Nx, fft_args(c, 0, nullptr)); }), "xconv");
-    HIPCHK(c, timed(c, "ypass_inv", 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, nullptr)); }), "ypass");
-    HIPCHK(c, timed(c, "zinv", spec_b + nre, [&] { return launch_zinv(c->Nz, fft_args(c, 2, nullptr)); }), "zinv");
+    HIPCHK(c, timed(c, "xconv", 2 * spec_b + pq, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }), "xconv");
+    HIPCHK(c, timed(c, "ypass_inv", 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, v)); }), "ypass");
+    HIPCHK(c, timed(c, "zinv", spec_b + nre, [&] { return launch_zinv(c->Nz, fft_args(c, 2, v)); }), "zinv");
     const double fb = 2.0 * (double)(c->Ny * c->Nz + c->Nx * c->Nz + c->Nx * c->Ny) * real_elem(c);
     HIPCHK(c, timed(c, "facefix", fb, [&] {
                return launch_facefix(c->stream, c->b.phi, c->prec == DBEC_MIXED, c->Nx, c->Ny, c->Nz);
@@ -158,25 +182,40 @@ int run_calcpsidd2(dbec_ctx *c, const double *scale) {
 
 // The composite step's convolution: the forward passes, then the inverse z
 // pass fused with calcnu (interior rows) and the face rows.
-int run_conv_nu(dbec_ctx *c, const double *scale) {
+int run_conv_nu(dbec_ctx *c, const Solves &v) {
+    const double nb = v.nb;
     const double ce = (double)(2 * real_elem(c));
     const double spec_b = (double)c->Nkz * c->Nx * c->Ny * ce;
-    HIPCHK(c, timed(c, "zfwd", (double)c->N * psi_elem(c) + spec_b, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, scale)); }), "zfwd");
-    HIPCHK(c, timed(c, "ypass_fwd", 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, nullptr)); }), "ypass");
+    HIPCHK(c, timed(c, "zfwd", nb * ((double)c->N * psi_elem(c) + spec_b), [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }), "zfwd");
+    HIPCHK(c, timed(c, "ypass_fwd", nb * 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }), "ypass");
     const double pq = (double)c->Nkz * (c->Nx / 2 + 1) * (c->Ny / 2 + 1) * real_elem(c);
-    HIPCHK(c, timed(c, "xconv", 2 * spec_b + pq, [&] { return launch_xconv(c->Nx, fft_args(c, 0, nullptr)); }), "xconv");
-    HIPCHK(c, timed(c, "ypass_inv", 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, nullptr)); }), "ypass");
+    HIPCHK(c, timed(c, "xconv", nb * (2 * spec_b + pq), [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }), "xconv");
+    HIPCHK(c, timed(c, "ypass_inv", nb * 2 * spec_b, [&] { return launch_ypass(c->Ny, +1, fft_args(c, 1, v)); }), "ypass");
     const double nf = (double)(c->Nx + c->Ny - 1);
-    HIPCHK(c, timed(c, "zinv_nu", spec_b + 2.0 * c->N * psi_elem(c) + nf * c->Nkz * 2 * ce,
-                    [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, scale)); }), "zinv_nu");
+    HIPCHK(c, timed(c, "zinv_nu", nb * (spec_b + 2.0 * c->N * psi_elem(c) + nf * c->Nkz * 2 * ce),
+                    [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }), "zinv_nu");
     c->phi_valid = false;
     return DBEC_OK;
 }
 
-int run_sweep(dbec_ctx *c, int axis, double *colpart) {
+int run_sweep(dbec_ctx *c, int axis, const Solves &v, bool cols) {
     static const char *names[3] = {"sweep_x", "sweep_y", "sweep_z"};
-    double bytes = 2.0 * (double)c->N * psi_elem(c) + (colpart ? (double)c->Nx * sweep_z_tiles(c) * 8.0 : 0.0);
-    HIPCHK(c, timed(c, names[axis], bytes, [&] { return launch_sweep(c->stream, c, axis, colpart); }), "sweep");
+    double bytes = v.nb * (2.0 * (double)c->N * psi_elem(c) + (cols ? (double)c->Nx * sweep_z_tiles(c) * 8.0 : 0.0));
+    HIPCHK(c, timed(c, names[axis], bytes, [&] { return launch_sweep(c->stream, c, axis, v, cols); }), "sweep");
+    return DBEC_OK;
+}
+
+// One split step (calcpsidd2, calcnu, calclux/y/z and, in imaginary time,
+// the norm whose 1/norm scale the next read applies) of the solves v.
+int run_step(dbec_ctx *c, const Solves &v) {
+    const bool imag = c->mode == DBEC_IMAG;
+    CHK(run_conv_nu(c, v));
+    CHK(run_sweep(c, 0, v, false));
+    CHK(run_sweep(c, 1, v, false));
+    CHK(run_sweep(c, 2, v, imag));
+    if (imag)
+        HIPCHK(c, timed(c, "norm", v.nb * (double)c->Nx * sweep_z_tiles(c) * 8.0,
+                        [&] { return launch_norm_from_cols(c->stream, c, v); }), "norm");
     return DBEC_OK;
 }
 
@@ -249,12 +288,6 @@ int dev_alloc(dbec_ctx *c, void **p, size_t bytes, const char *what) {
     }
     return DBEC_OK;
 }
-
-#define CHK(expr)                    \
-    do {                             \
-        int rc_ = (expr);            \
-        if (rc_ != DBEC_OK) return rc_; \
-    } while (0)
 
 int setup(dbec_ctx *c) {
     const dbec_params &p = c->p;
@@ -553,7 +586,7 @@ int dbec_calcpsidd2(dbec_ctx *c) {
     if (!c) return fail(c, DBEC_EINVAL, "null context");
     (void)hipSetDevice(c->device);
     CHK(materialize_scale(c));
-    return run_calcpsidd2(c, nullptr);
+    return run_calcpsidd2(c);
 }
 
 int dbec_get_psidd2(dbec_ctx *c, double *phi) {
@@ -582,7 +615,7 @@ static int sweep_api(dbec_ctx *c, int axis) {
     if (!c) return fail(c, DBEC_EINVAL, "null context");
     (void)hipSetDevice(c->device);
     CHK(materialize_scale(c));
-    return run_sweep(c, axis, nullptr);
+    return run_sweep(c, axis, solve_ref(c, false), false);
 }
 int dbec_calclux(dbec_ctx *c) { return sweep_api(c, 0); }
 int dbec_calcluy(dbec_ctx *c) { return sweep_api(c, 1); }
@@ -605,7 +638,7 @@ int dbec_calcmuen(dbec_ctx *c, double *mu, double *en) {
     if (!c || !mu || !en) return fail(c, DBEC_EINVAL, "null argument");
     (void)hipSetDevice(c->device);
     CHK(materialize_scale(c));
-    CHK(run_calcpsidd2(c, nullptr));
+    CHK(run_calcpsidd2(c));
     double *rmu = c->b.rows, *ren = c->b.rows + c->Nx * c->Ny;
     HIPCHK(c, launch_rows_muen(c->stream, c, rmu, ren), "muen rows");
     HIPCHK(c, launch_finalize(c->stream, c, 2, rmu, ren), "finalize");
@@ -660,17 +693,8 @@ int dbec_step(dbec_ctx *c, long nsteps, double g_stp, double gd_stp, double *nor
             c->g += g_stp;
             c->gd += gd_stp;
         }
-        const double *scale = c->scale_pending ? c->b.scal + SC_SCALE : nullptr;
-        CHK(run_conv_nu(c, scale));
-        c->scale_pending = false;
-        CHK(run_sweep(c, 0, nullptr));
-        CHK(run_sweep(c, 1, nullptr));
-        CHK(run_sweep(c, 2, imag ? c->b.rows : nullptr));
-        if (imag) {
-            HIPCHK(c, timed(c, "norm", (double)c->Nx * sweep_z_tiles(c) * 8.0,
-                            [&] { return launch_norm_from_cols(c->stream, c, c->b.rows); }), "norm");
-            c->scale_pending = true;
-        }
+        CHK(run_step(c, solve_ref(c, c->scale_pending)));
+        c->scale_pending = imag;
     }
     if (imag && nsteps > 0 && norm) {
         CHK(materialize_scale(c));

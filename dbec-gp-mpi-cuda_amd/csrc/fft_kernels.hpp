@@ -72,17 +72,19 @@ struct PsiTraits<cpx<float>> {
 // ---------------------------------------------------------------------------
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__restrict__ spec,
-       int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
+    const SolveAt sr = solve_at(v);
+    const P *__restrict__ psi = reinterpret_cast<const P *>(sr.psi);
+    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(sr.spec);
     const int tid = threadIdx.x;
     const int ytiles = (Ny + 2 * L - 1) / (2 * L);
     const int x = blockIdx.x / ytiles;
     const int y0 = (blockIdx.x % ytiles) * 2 * L;
     const int nrows = min(2 * L, Ny - y0);
-    const double s = scale ? *scale : 1.0;
+    const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
     // Rows 2p and 2p+1 of the tile become the real and imaginary parts of
     // line p.  A thread takes VE consecutive z of both rows, so its LDS
@@ -137,10 +139,11 @@ k_zfwd(const P *__restrict__ psi, const double *__restrict__ scale, cpx<R> *__re
 // ---------------------------------------------------------------------------
 template <int N, int DIR, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) {
+k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
+    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(solve_at(v).spec);
     const int tid = threadIdx.x;
     stage_twiddles(tw, twg, N, tid, WG);
     const long line0 = (long)blockIdx.x * L;
@@ -171,12 +174,13 @@ k_ypass(cpx<R> *__restrict__ spec, long nlines, const cpx<R> *__restrict__ twg) 
 // ---------------------------------------------------------------------------
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_xconv(cpx<R> *__restrict__ spec, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
+k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L;
     constexpr int NXH = N / 2 + 1;
     constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
+    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(solve_at(v).spec);
     const int tid = threadIdx.x;
     const int ytiles = (Ny + L - 1) / L;
     const int kz = blockIdx.x / ytiles;
@@ -333,8 +337,7 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 // potential are staged in LDS.
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__restrict__ scale,
-          const double *__restrict__ sq, TrapConst tc, NuConst nc, int Nx, int Ny, int nfaceb,
+k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, int nfaceb,
           const cpx<R> *__restrict__ twg) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     constexpr int Q = 2 * L * N / WG;  // psi elements per thread (interior rows)
@@ -345,8 +348,14 @@ k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__
     __shared__ cpx<R> tw[N];
     __shared__ double sqz[N];
     __shared__ double sqr[2 * L];  // interior: y^2 of the tile's rows; faces: x^2 | y^2 of each face row
+    const SolveAt sr = solve_at(v);
+    const cpx<R> *__restrict__ spec = reinterpret_cast<const cpx<R> *>(sr.spec);
+    P *__restrict__ psi = reinterpret_cast<P *>(sr.psi);
+    const TrapConst tc = sr.p.tc;
+    nc.g = sr.p.g;
+    nc.gd = sr.p.gd;
     const int tid = threadIdx.x;
-    const double s = scale ? *scale : 1.0;
+    const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
     for (int i = tid; i < N; i += WG) sqz[i] = sq[Nx + Ny + i];
     if ((int)blockIdx.x >= nfaceb) {
@@ -449,8 +458,9 @@ k_zinv_nu(const cpx<R> *__restrict__ spec, P *__restrict__ psi, const double *__
 }
 
 // ---------------------------------------------------------------------------
-// Launchers for one length.  The z forward pass runs with tiles twice as
-// wide (measured faster at 256^3); DBEC_FFT_LM_ZFWD=1 selects the narrow one.
+// Launchers for one length.  Each pass has a tile multiplier LM (1 or 2)
+// chosen per precision from measurements at 256^3; the environment variables
+// DBEC_FFT_LM_{ZFWD,YPASS,XCONV,FUSED}{64,32} override them (tuning runs).
 inline int fft_lm(const char *name, int dflt) {
     const char *e = getenv(name);
     if (!e) return dflt;
@@ -464,31 +474,22 @@ struct FftLaunch {
     static hipError_t zfwd_t(const FftArgs &a) {
         using Tl = FftTile<N, LM>;
         const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
-        dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
+        dim3 grid((unsigned)(a.Nx * ytiles), (unsigned)a.v.nb), block(Tl::WG);
+        const int Nx = (int)a.Nx, Ny = (int)a.Ny;
         if (!a.fp32) {
-            auto *spec = static_cast<cpx<double> *>(a.spec);
             auto *tw = static_cast<const cpx<double> *>(a.tw);
-            if (a.cpsi)
-                k_zfwd<N, double, cpx<double>, LM><<<grid, block, 0, a.stream>>>(
-                    static_cast<const cpx<double> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
-            else
-                k_zfwd<N, double, double, LM><<<grid, block, 0, a.stream>>>(
-                    static_cast<const double *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+            if (a.cpsi) k_zfwd<N, double, cpx<double>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
+            else k_zfwd<N, double, double, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
         } else {
-            auto *spec = static_cast<cpx<float> *>(a.spec);
             auto *tw = static_cast<const cpx<float> *>(a.tw);
-            if (a.cpsi)
-                k_zfwd<N, float, cpx<float>, LM><<<grid, block, 0, a.stream>>>(
-                    static_cast<const cpx<float> *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
-            else
-                k_zfwd<N, float, float, LM><<<grid, block, 0, a.stream>>>(
-                    static_cast<const float *>(a.psi), a.scale, spec, (int)a.Nx, (int)a.Ny, tw);
+            if (a.cpsi) k_zfwd<N, float, cpx<float>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
+            else k_zfwd<N, float, float, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
         }
         return hipGetLastError();
     }
     static hipError_t zfwd(const FftArgs &a) {
-        static const int lm = fft_lm("DBEC_FFT_LM_ZFWD", 2);
-        return lm == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
+        static const int l64 = fft_lm("DBEC_FFT_LM_ZFWD64", 1), l32 = fft_lm("DBEC_FFT_LM_ZFWD32", 2);
+        return (a.fp32 ? l32 : l64) == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
     }
     // FP32 spectra use tiles twice as wide, so every strided access still
     // moves 128-256 contiguous bytes.
@@ -497,49 +498,52 @@ struct FftLaunch {
     static hipError_t ypass_t(int dir, const FftArgs &a) {
         using Tl = FftTile<N, LM>;
         const long nlines = a.Nkz * a.Nx;
-        dim3 grid((unsigned)((nlines + Tl::L - 1) / Tl::L)), block(Tl::WG);
+        dim3 grid((unsigned)((nlines + Tl::L - 1) / Tl::L), (unsigned)a.v.nb), block(Tl::WG);
         if (!a.fp32) {
-            auto *spec = static_cast<cpx<double> *>(a.spec);
             auto *tw = static_cast<const cpx<double> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, double, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
-            else k_ypass<N, +1, double, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
+            else k_ypass<N, +1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
         } else {
-            auto *spec = static_cast<cpx<float> *>(a.spec);
             auto *tw = static_cast<const cpx<float> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, float, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
-            else k_ypass<N, +1, float, LM><<<grid, block, 0, a.stream>>>(spec, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
+            else k_ypass<N, +1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
         }
         return hipGetLastError();
     }
-    static hipError_t ypass(int dir, const FftArgs &a) { return a.fp32 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a); }
+    static hipError_t ypass(int dir, const FftArgs &a) {
+        static const int l64 = fft_lm("DBEC_FFT_LM_YPASS64", 1), l32 = fft_lm("DBEC_FFT_LM_YPASS32", 2);
+        return (a.fp32 ? l32 : l64) == 2 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a);
+    }
 
     template <int LM>
     static hipError_t xconv_t(const FftArgs &a) {
         using Tl = FftTile<N, LM>;
         const int ytiles = (int)((a.Ny + Tl::L - 1) / Tl::L);
-        dim3 grid((unsigned)(a.Nkz * ytiles)), block(Tl::WG);
+        dim3 grid((unsigned)(a.Nkz * ytiles), (unsigned)a.v.nb), block(Tl::WG);
         if (!a.fp32)
-            k_xconv<N, double, LM><<<grid, block, 0, a.stream>>>(static_cast<cpx<double> *>(a.spec),
-                                                                 static_cast<const double *>(a.potddq), (int)a.Ny,
-                                                                 static_cast<const cpx<double> *>(a.tw));
+            k_xconv<N, double, LM><<<grid, block, 0, a.stream>>>(a.v, static_cast<const double *>(a.potddq),
+                                                                 (int)a.Ny, static_cast<const cpx<double> *>(a.tw));
         else
-            k_xconv<N, float, LM><<<grid, block, 0, a.stream>>>(static_cast<cpx<float> *>(a.spec),
-                                                                static_cast<const float *>(a.potddq), (int)a.Ny,
-                                                                static_cast<const cpx<float> *>(a.tw));
+            k_xconv<N, float, LM><<<grid, block, 0, a.stream>>>(a.v, static_cast<const float *>(a.potddq),
+                                                                (int)a.Ny, static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
     }
-    static hipError_t xconv(const FftArgs &a) { return a.fp32 ? xconv_t<2>(a) : xconv_t<1>(a); }
+    static hipError_t xconv(const FftArgs &a) {
+        static const int l64 = fft_lm("DBEC_FFT_LM_XCONV64", 1), l32 = fft_lm("DBEC_FFT_LM_XCONV32", 2);
+        return (a.fp32 ? l32 : l64) == 2 ? xconv_t<2>(a) : xconv_t<1>(a);
+    }
 
+    // single solve: spectrum of `one` -> phi
     static hipError_t zinv(const FftArgs &a) {
         using Tl = FftTile<N, 1>;
         const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
         dim3 grid((unsigned)(a.Nx * ytiles)), block(Tl::WG);
         if (!a.fp32)
-            k_zinv<N, double, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<double> *>(a.spec),
+            k_zinv<N, double, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<double> *>(a.v.spec),
                                                                static_cast<double *>(a.phi), (int)a.Nx, (int)a.Ny,
                                                                static_cast<const cpx<double> *>(a.tw));
         else
-            k_zinv<N, float, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<float> *>(a.spec),
+            k_zinv<N, float, 1><<<grid, block, 0, a.stream>>>(static_cast<const cpx<float> *>(a.v.spec),
                                                               static_cast<float *>(a.phi), (int)a.Nx, (int)a.Ny,
                                                               static_cast<const cpx<float> *>(a.tw));
         return hipGetLastError();
@@ -548,15 +552,12 @@ struct FftLaunch {
     template <typename P, typename R, int LM>
     static hipError_t fused_t(const FftArgs &a) {
         using Tl = FftTile<N, LM>;
-        const auto *spec = static_cast<const cpx<R> *>(a.spec);
-        auto *psi = static_cast<P *>(a.psi);
         const auto *tw = static_cast<const cpx<R> *>(a.tw);
         const int ytiles = (int)((a.Ny + 2 * Tl::L - 1) / (2 * Tl::L));
         const int nmain = (int)((a.Nx - 1) * ytiles);
         const int nfaceb = (int)((a.Ny + a.Nx - 1 + Tl::L - 1) / Tl::L);
-        dim3 grid((unsigned)(nmain + nfaceb)), block(Tl::WG);
-        k_zinv_nu<N, R, P, LM><<<grid, block, 0, a.stream>>>(spec, psi, a.scale, a.sq, a.tc, a.nc, (int)a.Nx,
-                                                             (int)a.Ny, nfaceb, tw);
+        dim3 grid((unsigned)(nmain + nfaceb), (unsigned)a.v.nb), block(Tl::WG);
+        k_zinv_nu<N, R, P, LM><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny, nfaceb, tw);
         return hipGetLastError();
     }
     template <int LM>
@@ -564,7 +565,12 @@ struct FftLaunch {
         if (!a.fp32) return a.cpsi ? fused_t<cpx<double>, double, LM>(a) : fused_t<double, double, LM>(a);
         return a.cpsi ? fused_t<cpx<float>, float, LM>(a) : fused_t<float, float, LM>(a);
     }
-    static hipError_t fused(const FftArgs &a) { return a.fp32 ? fused_lm<2>(a) : fused_lm<1>(a); }
+    static hipError_t fused(const FftArgs &a) {
+        // FP64: wide tiles for real psi, narrow ones for complex psi
+        static const int l64 = fft_lm("DBEC_FFT_LM_FUSED64", 0), l32 = fft_lm("DBEC_FFT_LM_FUSED32", 2);
+        const int lm = a.fp32 ? l32 : (l64 ? l64 : (a.cpsi ? 1 : 2));
+        return lm == 2 ? fused_lm<2>(a) : fused_lm<1>(a);
+    }
 };
 
 }  // namespace dbec

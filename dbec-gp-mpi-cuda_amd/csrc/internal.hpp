@@ -70,21 +70,61 @@ struct dbec_ctx {
 
 namespace dbec {
 
+// ---- The solves of a launch ----------------------------------------------
+// The step kernels advance nb solves of one grid at once: gridDim.y = nb and
+// block row y works on solve slot[y] (slot y when `slot` is null).  Slot s
+// owns the buffers at base + s * stride, so every buffer address is kernel
+// argument arithmetic (global address space, no pointer loads).  Solves of one
+// launch share the grid, dx, dt, cutoff and precision, hence every table;
+// they differ in their buffers, trap and couplings.  No kernel mixes data of
+// two solves, so a solve's result does not depend on the others in its launch.
+struct SolveParams {
+    TrapConst tc;
+    double g, gd;  // couplings of this step
+};
+struct Solves {
+    void *psi;
+    void *spec;
+    double *rows;  // z-sweep column partials of psi^2 (imaginary time)
+    double *scal;  // device scalars, SC_* slots
+    long psi_st, spec_st, rows_st, scal_st;  // bytes from one slot to the next
+    const int *slot;          // launch row -> slot, or nullptr
+    const SolveParams *par;   // per slot, or nullptr: every row uses p0
+    SolveParams p0;
+    int nb;                   // launch rows (gridDim.y)
+    int use_scale;            // psi is read as psi * scal[SC_SCALE] (pending renormalisation)
+};
+
+struct SolveAt {
+    char *psi, *spec;
+    double *rows, *scal;
+    SolveParams p;
+};
+__device__ __forceinline__ SolveAt solve_at(const Solves &v) {
+    const int y = blockIdx.y;
+    const long sl = v.slot ? v.slot[y] : y;
+    SolveAt r;
+    r.psi = static_cast<char *>(v.psi) + sl * v.psi_st;
+    r.spec = static_cast<char *>(v.spec) + sl * v.spec_st;
+    r.rows = reinterpret_cast<double *>(reinterpret_cast<char *>(v.rows) + sl * v.rows_st);
+    r.scal = reinterpret_cast<double *>(reinterpret_cast<char *>(v.scal) + sl * v.scal_st);
+    if (v.par) r.p = v.par[sl];
+    else r.p = v.p0;
+    return r;
+}
+
 // ---- FFT passes (fft_kernels.hpp, one instantiation per length) ----------
 struct FftArgs {
     hipStream_t stream;
     long Nx, Ny, Nz, Nkz;
     bool fp32;      // spec / phi / psi stored in FP32
     bool cpsi;      // psi complex
-    void *psi;
-    const double *scale;  // multiply psi by *scale while reading (imaginary time), or nullptr
-    void *spec;
-    void *phi;
+    Solves v;             // the solves of the launch
+    void *phi;            // standalone calcpsidd2 (single solve): phi output
     const void *potddq;
     const void *tw;       // twiddles of the transformed axis (FP64 or FP32)
     const double *sq;     // x^2 | y^2 | z^2 (fused calcnu)
-    TrapConst tc;
-    NuConst nc;
+    NuConst nc;           // dt and grid size; couplings come from the solve
 };
 // Each returns hipSuccess or an error; `n` is the transformed length.
 hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C along z)

@@ -323,11 +323,12 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
 // (l, s) loads its segment straight from HBM into registers.
 template <typename S, int AXIS, int L, int M>
 __global__ void __launch_bounds__(512)
-k_sweep_xy(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
-           const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
-           SweepConst<typename Arith<S>::T> k) {
+k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
+           const typename Arith<S>::T *__restrict__ calpha, const typename Arith<S>::T *__restrict__ cgamma,
+           const typename Arith<S>::T *__restrict__ ga, SweepConst<typename Arith<S>::T> k) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
+    S *__restrict__ psi = reinterpret_cast<S *>(solve_at(sv).psi);
     const int n = AXIS == 0 ? Nx : Ny;
     const int P = n / M;
     const int nt = L * P;
@@ -338,24 +339,19 @@ k_sweep_xy(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>:
     const int pl = blockIdx.x / zt, z0 = (blockIdx.x % zt) * L;
     const int nl = min(L, Nz - z0);
     const bool act = l < nl;
-    // lanes past the end of a partial tile read the last line (and discard it)
-    const long base = (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + (act ? l : nl - 1);
+    const long base = (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + l;
     const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
     const int e0 = s * M;
     const A zero = m_zero(k.A);
     const CoefPre<A> cp = coef_pre(calpha, cgamma, ga, n, tid);
-    // all loads in flight before any conversion
-    S raw[M];
-#pragma unroll
-    for (int j = 0; j < M; ++j) raw[j] = psi[base + (long)(e0 + j) * es];
-    const S rl = psi[base + (long)(e0 > 0 ? e0 - 1 : 0) * es];
-    const S rr = psi[base + (long)(e0 + M < n ? e0 + M : n - 1) * es];
-    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
+    // Predicated loads converted in place: fewer registers (higher occupancy)
+    // than issuing them into a separate array, which measured faster here.
     A v[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? to_a(raw[j]) : zero;
-    const A hl = (act && e0 > 0) ? to_a(rl) : zero;
-    const A hr = (act && e0 + M < n) ? to_a(rr) : zero;
+    for (int j = 0; j < M; ++j) v[j] = act ? to_a(psi[base + (long)(e0 + j) * es]) : zero;
+    const A hl = (act && e0 > 0) ? to_a(psi[base + (long)(e0 - 1) * es]) : zero;
+    const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
+    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
     __syncthreads();
     seg_solve<A, M, false>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
@@ -373,12 +369,15 @@ k_sweep_xy(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>:
 // colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
 template <typename S, int L, int M, bool COLS>
 __global__ void __launch_bounds__(512)
-k_sweep_z(S *__restrict__ psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
-          const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
-          SweepConst<typename Arith<S>::T> k, double *__restrict__ colpart, const double *__restrict__ wz,
-          const double *__restrict__ wy) {
+k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
+          const typename Arith<S>::T *__restrict__ calpha, const typename Arith<S>::T *__restrict__ cgamma,
+          const typename Arith<S>::T *__restrict__ ga, SweepConst<typename Arith<S>::T> k,
+          const double *__restrict__ wz, const double *__restrict__ wy) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
+    const SolveAt sr = solve_at(sv);
+    S *__restrict__ psi = reinterpret_cast<S *>(sr.psi);
+    double *__restrict__ colpart = sr.rows;
     const int n = Nz;
     const int RS = n + 1;  // row stride of the tile
     const int P = n / M;
@@ -472,12 +471,18 @@ static int seg_len(long n) {
 // 512 threads.
 static int z_seg_len(long) { return 16; }
 
+static int env_int(const char *name, int dflt) {
+    const char *e = getenv(name);
+    return e ? atoi(e) : dflt;
+}
+
 template <typename S>
 static int z_rows(long n) {
+    static const int max_threads = env_int("DBEC_SWEEP_Z_THREADS", 256);
     const int M = z_seg_len(n);
     const int P = (int)(n / M);
     int L = 64;
-    while (L > 4 && ((long)L * P > 512 || (long)(n + 1) * L * (long)sizeof(S) > 49152)) L /= 2;
+    while (L > 4 && ((long)L * P > max_threads || (long)(n + 1) * L * (long)sizeof(S) > 49152)) L /= 2;
     return L;
 }
 
@@ -491,8 +496,14 @@ long sweep_z_tiles(const dbec_ctx *c) {
     return (c->Ny + L - 1) / L;
 }
 
+// Solves of one launch: the by-value `one` or nb entries of `many`.
+struct Launch {
+    hipStream_t s;
+    Solves v;
+};
+
 template <typename S, int AXIS>
-static hipError_t sweep_xy_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
                                     const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
                                     const SweepConst<typename Arith<S>::T> &k) {
     using A = typename Arith<S>::T;
@@ -503,61 +514,59 @@ static hipError_t sweep_xy_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int N
     const int P = n / M;
     if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
     const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
-    dim3 grid((unsigned)nblocks), block(L * P);
+    dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
     const size_t lds = SweepShared<A>::bytes(n, L * P, L);
     if constexpr (sizeof(A) == 8) {
         if (M == 32) {
-            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k);
+            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
             return hipGetLastError();
         }
     }
-    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k);
+    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
     return hipGetLastError();
 }
 
 template <typename S, int L, bool COLS>
-static hipError_t sweep_z_launch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
                                  const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
-                                 const SweepConst<typename Arith<S>::T> &k, double *colpart, const double *wz,
-                                 const double *wy) {
+                                 const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
     using A = typename Arith<S>::T;
     const int P = Nz / 16;
     const long nblocks = (long)((Ny + L - 1) / L) * Nx;
-    dim3 grid((unsigned)nblocks), block(L * P);
+    dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
     const size_t lds = ((SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15)) + (size_t)(Nz + 1) * L * sizeof(S);
-    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, s>>>(psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
     return hipGetLastError();
 }
 
 template <typename S, bool COLS>
-static hipError_t sweep_z_dispatch(hipStream_t s, S *psi, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+static hipError_t sweep_z_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
                                    const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
-                                   const SweepConst<typename Arith<S>::T> &k, double *colpart, const double *wz,
-                                   const double *wy) {
-    using A = typename Arith<S>::T;
+                                   const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
     if (Nz % 16 != 0) return hipErrorInvalidValue;
     switch (z_rows<S>(Nz)) {
-    case 64: return sweep_z_launch<S, 64, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
-    case 32: return sweep_z_launch<S, 32, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
-    case 16: return sweep_z_launch<S, 16, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
-    case 8: return sweep_z_launch<S, 8, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
-    default: return sweep_z_launch<S, 4, COLS>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
+    case 64: return sweep_z_launch<S, 64, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    case 32: return sweep_z_launch<S, 32, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    case 16: return sweep_z_launch<S, 16, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    case 8: return sweep_z_launch<S, 8, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    default: return sweep_z_launch<S, 4, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
     }
 }
 
 template <typename S, typename A>
-static hipError_t sweep_axis(hipStream_t s, S *psi, const dbec_ctx *c, int axis, const A *ca, const A *cg,
-                             const A *ga, const SweepConst<A> &k, double *colpart) {
+static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const A *ca, const A *cg, const A *ga,
+                             const SweepConst<A> &k, bool cols) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
-    if (axis == 0) return sweep_xy_dispatch<S, 0>(s, psi, Nx, Ny, Nz, ca, cg, ga, k);
-    if (axis == 1) return sweep_xy_dispatch<S, 1>(s, psi, Nx, Ny, Nz, ca, cg, ga, k);
-    if (colpart) return sweep_z_dispatch<S, true>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, colpart, wz, wy);
-    return sweep_z_dispatch<S, false>(s, psi, Nx, Ny, Nz, ca, cg, ga, k, nullptr, wz, wy);
+    if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, ca, cg, ga, k);
+    if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, ca, cg, ga, k);
+    if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
 }
 
-hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colpart) {
+hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves &v, bool cols) {
     const auto &b = c->b;
+    const Launch q{s, v};
     const long off[3] = {0, c->Nx - 1, c->Nx - 1 + c->Ny - 1};
     const long tot = (c->Nx - 1) + (c->Ny - 1) + (c->Nz - 1);
     const bool f32 = c->prec == DBEC_MIXED;
@@ -569,8 +578,8 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colp
         k.A0r = c->A0r[axis][0];
         k.A = c->A[axis][0];
         k.mA = -c->A[axis][0];
-        return f32 ? sweep_axis(s, static_cast<float *>(b.psi), c, axis, ca, cg, ga, k, colpart)
-                   : sweep_axis(s, static_cast<double *>(b.psi), c, axis, ca, cg, ga, k, colpart);
+        return f32 ? sweep_axis<float>(q, c, axis, ca, cg, ga, k, cols)
+                   : sweep_axis<double>(q, c, axis, ca, cg, ga, k, cols);
     }
     const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
     const cpx<double> *ca = cc + off[axis];
@@ -580,15 +589,17 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colp
     k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
     k.A = {c->A[axis][0], c->A[axis][1]};
     k.mA = {c->mA[axis][0], c->mA[axis][1]};
-    return f32 ? sweep_axis(s, static_cast<cpx<float> *>(b.psi), c, axis, ca, cg, ga, k, colpart)
-               : sweep_axis(s, static_cast<cpx<double> *>(b.psi), c, axis, ca, cg, ga, k, colpart);
+    return f32 ? sweep_axis<cpx<float>>(q, c, axis, ca, cg, ga, k, cols)
+               : sweep_axis<cpx<double>>(q, c, axis, ca, cg, ga, k, cols);
 }
 
 // norm^2 = sum_x wx[x] sum_t colpart[t][x]; scal[NORM] = norm, scal[SCALE] = 1/norm.
 __global__ void __launch_bounds__(256)
-k_norm_cols(const double *__restrict__ colpart, int tiles, const double *__restrict__ wx, int Nx,
-            double *__restrict__ scal) {
+k_norm_cols(Solves v, int tiles, const double *__restrict__ wx, int Nx) {
     __shared__ double sh[256];
+    const SolveAt sr = solve_at(v);
+    const double *__restrict__ colpart = sr.rows;
+    double *__restrict__ scal = sr.scal;
     const int tid = threadIdx.x;
     double p = 0.0;
     for (int x = tid; x < Nx; x += 256) {
@@ -609,8 +620,8 @@ k_norm_cols(const double *__restrict__ colpart, int tiles, const double *__restr
     }
 }
 
-hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const double *colpart) {
-    k_norm_cols<<<1, 256, 0, s>>>(colpart, (int)sweep_z_tiles(c), c->b.wgt, (int)c->Nx, c->b.scal);
+hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const Solves &v) {
+    k_norm_cols<<<dim3(1, (unsigned)v.nb), 256, 0, s>>>(v, (int)sweep_z_tiles(c), c->b.wgt, (int)c->Nx);
     return hipGetLastError();
 }
 

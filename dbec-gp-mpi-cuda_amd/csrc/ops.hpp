@@ -20,13 +20,15 @@ struct SweepConst {
 hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny, long Nz);
 hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale);
 hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale);
-// z sweep with colpart != nullptr also writes, per (y tile t, x),
-// colpart[t][x] = sum over the tile's rows of wy[y] * (Simpson z integral of psi^2).
-hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, double *colpart);
-// number of y tiles of the z sweep (rows of colpart)
+// Sweep along `axis` of the solves v.
+// The grid, coefficients and weights come from c.  The z sweep with cols also
+// writes, per solve and (y tile t, x), rows[t][x] = sum over the tile's rows
+// of wy[y] * (Simpson z integral of psi^2).
+hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves &v, bool cols);
+// number of y tiles of the z sweep (rows of the column partials)
 long sweep_z_tiles(const dbec_ctx *c);
-// norm from the z sweep's column partials: scal[NORM], scal[SCALE]
-hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const double *colpart);
+// norm of each solve from its column partials: scal[NORM], scal[SCALE]
+hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const Solves &v);
 TrapConst trap_const(const dbec_ctx *c);
 NuConst nu_const(const dbec_ctx *c);
 hipError_t launch_rows_mod2(hipStream_t s, const dbec_ctx *c, double *rows0, double *rowsz);
