@@ -86,6 +86,62 @@ def copy_calibration(nbytes):
     hip.hipFree(b)
 
 
+def percentiles(xs):
+    import numpy as np
+    a = np.asarray(xs, dtype=float)
+    return {"p50": float(np.percentile(a, 50)), "p95": float(np.percentile(a, 95)),
+            "p99": float(np.percentile(a, 99))}
+
+
+def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeout):
+    """The serving stream (first `count` requests, Poisson arrivals at `rate`
+    per second, mixed precision) replayed in real time against the request
+    loop, and its first `baseline_count` requests as back-to-back plain FP64
+    solves (one at a time through dbec.h, arrivals queued FIFO)."""
+    import numpy as np
+    import dbecpy
+    import dbecserve as ds
+
+    stream = ds.stream_requests(count, rate=rate, seed=0, prec=dbecpy.MIXED)
+    with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every) as srv:
+        t0 = time.perf_counter()
+        res = ds.serve_stream(srv, stream, timeout=timeout)
+        wall = time.perf_counter() - t0
+    lat = [r["latency"] for r in res]
+    status = {}
+    for r in res:
+        status[r["status"]] = status.get(r["status"], 0) + 1
+    out = {
+        "requests": count, "rate_per_s": rate, "precision": "mixed", "max_batch": max_batch,
+        "check_every": check_every, "wall_s": wall, "solves_per_s": count / wall,
+        "latency_s": percentiles(lat), "mean_steps": float(np.mean([r["steps"] for r in res])),
+        "status_counts": {["converged", "steps_done", "not_converged", "failed"][k]: v for k, v in status.items()},
+    }
+    log(f"serving: {count} requests in {wall:.2f} s = {count / wall:.2f} solves/s, "
+        f"p50/p95/p99 {out['latency_s']['p50']:.2f}/{out['latency_s']['p95']:.2f}/{out['latency_s']['p99']:.2f} s")
+    if baseline_count:
+        sub = stream[:baseline_count]
+        fp64 = [ds.make_request(r.id, r.nx, r.mode, dbecpy.FP64, r.g / r.par, r.gd / r.par,
+                                (r.vgamma, r.vnu, r.vlambda), r.dt, r.max_steps, r.mutol) for _, r in sub]
+        service = []
+        for q in fp64:
+            t1 = time.perf_counter()
+            ds.plain_solve(lib, q, check_every=check_every)
+            service.append(time.perf_counter() - t1)
+        # FIFO single server: start = max(arrival, previous finish)
+        fin, blat = 0.0, []
+        for (arr, _), sv in zip(sub, service):
+            fin = max(arr, fin) + sv
+            blat.append(fin - arr)
+        out["baseline_back_to_back_fp64"] = {
+            "requests": baseline_count, "solves_per_s": baseline_count / sum(service),
+            "latency_s": percentiles(blat), "mean_service_s": float(np.mean(service)),
+        }
+        out["speedup_solves_per_s"] = out["solves_per_s"] / out["baseline_back_to_back_fp64"]["solves_per_s"]
+        log(f"back-to-back FP64: {baseline_count} requests, {baseline_count / sum(service):.2f} solves/s")
+    return out
+
+
 def traffic_for(workload, kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (profiles/traffic_<workload>.json), or None."""
@@ -108,6 +164,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=0,
+                    help="also time the same grid in mixed precision for this many steps (imag256 only)")
+    ap.add_argument("--serve-requests", type=int, default=0, help="requests of the serving stream to replay")
+    ap.add_argument("--serve-baseline", type=int, default=0, help="of those, run back to back in plain FP64")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-batch", type=int, default=16)
+    ap.add_argument("--serve-check", type=int, default=50)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -197,6 +260,25 @@ def main():
         "observables": {"norm": norm, "mu_over_par": mu / 2.0, "E_over_par": en / 2.0},
         "cpu_baseline": None,
     }
+    if args.mixed_steps and mode == "imag" and prec == "fp64":
+        sm = dbecpy.Solver(lib, p, m, dbecpy.MIXED)
+        sm.set_psi(dbecpy.gaussian_psi0(p))
+        sm.calcnorm(rescale=True)
+        sm.set_coupling(80.0, 60.0)
+        sm.step(args.warmup, want_norm=False)
+        sm.synchronize()
+        t0 = time.perf_counter()
+        sm.step(args.mixed_steps, want_norm=False)
+        sm.synchronize()
+        tm = time.perf_counter() - t0
+        sm.close()
+        line["mixed"] = {"value": npts * args.mixed_steps / tm, "unit": "grid-pt-steps/s", "steps": args.mixed_steps,
+                         "ms_per_step": 1e3 * tm / args.mixed_steps, "speedup_vs_fp64": (npts * args.mixed_steps / tm) / value,
+                         "dtype": "f32+f64"}
+        log(f"mixed: {1e3 * tm / args.mixed_steps:.3f} ms/step, {line['mixed']['speedup_vs_fp64']:.2f}x FP64")
+    if args.serve_requests:
+        line["serving"] = serve_bench(lib, args.serve_requests, args.serve_baseline, args.serve_rate,
+                                      args.serve_batch, args.serve_check, timeout=600.0)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle, {args.cpu_steps} steps at {n}, {threads} threads")

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "core.hpp"
 #include "fft_kernels.hpp"
 #include "ops.hpp"
 
@@ -138,6 +139,7 @@ Solves solve_ref(const dbec_ctx *c, bool use_scale) {
     v.p0.gd = c->gd;
     v.nb = 1;
     v.use_scale = use_scale ? 1 : 0;
+    v.shift = (c->prec == DBEC_MIXED && c->mode == DBEC_IMAG) ? 1 : 0;
     return v;
 }
 
@@ -530,7 +532,7 @@ int dbec_destroy(dbec_ctx *c) {
     if (!c) return DBEC_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    free_buffers(c);
+    if (!c->view) free_buffers(c);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DBEC_OK;
@@ -546,6 +548,8 @@ int dbec_set_psi(dbec_ctx *c, const double *psi) {
         std::vector<float> f(psi, psi + nd);
         HIPCHK(c, hipMemcpyAsync(c->b.psi, f.data(), nd * sizeof(float), hipMemcpyHostToDevice, c->stream), "copy psi");
     }
+    const double zero = 0.0;  // a new psi starts without exponent shift
+    HIPCHK(c, hipMemcpyAsync(c->b.scal + SC_SHIFT, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream), "copy");
     HIPCHK(c, hipStreamSynchronize(c->stream), "synchronize");
     c->scale_pending = false;
     return DBEC_OK;
@@ -885,3 +889,33 @@ int dbec_profile_step(dbec_ctx *c, int reps, dbec_kernel_stat *stats, int max, i
 }
 
 }  // extern "C"
+
+namespace dbec {
+
+Solves core_solves(const dbec_ctx *c, bool use_scale) { return solve_ref(c, use_scale); }
+
+int core_step(dbec_ctx *c, const Solves &v) { return run_step(c, v); }
+
+int core_observe_async(dbec_ctx *c, int what) {
+    if (c->mode == DBEC_IMAG) {
+        HIPCHK(c, launch_scale(c->stream, c, c->b.scal + SC_SCALE), "scale");
+        HIPCHK(c, launch_set_scalar(c->stream, c->b.scal + SC_SCALE, 1.0), "scalar");
+    }
+    double *r0 = c->b.rows, *r1 = c->b.rows + c->Nx * c->Ny;
+    if (what & OBS_MUEN) {
+        CHK(run_calcpsidd2(c));
+        HIPCHK(c, launch_rows_muen(c->stream, c, r0, r1), "muen rows");
+        HIPCHK(c, launch_finalize(c->stream, c, 2, r0, r1), "finalize");
+    }
+    if (what & OBS_RMS) {
+        HIPCHK(c, launch_rows_mod2(c->stream, c, r0, r1), "rms rows");
+        HIPCHK(c, launch_finalize(c->stream, c, 1, r0, r1), "finalize");
+    }
+    return DBEC_OK;
+}
+
+int core_fail(dbec_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+size_t core_psi_elem(const dbec_ctx *c) { return psi_elem(c); }
+size_t core_real_elem(const dbec_ctx *c) { return real_elem(c); }
+
+}  // namespace dbec

@@ -354,6 +354,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     const TrapConst tc = sr.p.tc;
     nc.g = sr.p.g;
     nc.gd = sr.p.gd;
+    nc.shift = v.shift ? sr.scal[SC_SHIFT] : 0.0;
     const int tid = threadIdx.x;
     const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);

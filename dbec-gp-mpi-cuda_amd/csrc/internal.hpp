@@ -41,7 +41,10 @@ struct Buffers {
 };
 
 // Device scalar slots.
-enum { SC_NORM = 0, SC_SCALE = 1, SC_ONE = 2, SC_MU = 3, SC_EN = 4, SC_R0 = 5, SC_RX = 6, SC_RY = 7, SC_RZ = 8, SC_COUNT = 16 };
+// SC_SHIFT: exponent shift of the mixed-precision imaginary-time step (see
+// k_norm_cols), 0 elsewhere.
+enum { SC_NORM = 0, SC_SCALE = 1, SC_ONE = 2, SC_MU = 3, SC_EN = 4, SC_R0 = 5, SC_RX = 6, SC_RY = 7, SC_RZ = 8,
+       SC_SHIFT = 9, SC_LASTNORM = 10, SC_COUNT = 16 };
 
 struct KernelTimer;
 
@@ -62,6 +65,7 @@ struct dbec_ctx {
     std::vector<double> h_calpha[3], h_cgamma[3];  // host copies (1 or 2 doubles per entry)
     std::vector<double> h_potdd_half;              // [kx][ky][kz<=Nz/2], for dbec_get_table
     bool scale_pending = false;  // psi must be multiplied by scal[SC_SCALE]
+    bool view = false;           // buffers belong to another owner (request-loop slot)
     bool phi_valid = false;
     dbec::Buffers b;
     dbec::KernelTimer *timer = nullptr;  // set while dbec_profile_step records
@@ -93,6 +97,7 @@ struct Solves {
     SolveParams p0;
     int nb;                   // launch rows (gridDim.y)
     int use_scale;            // psi is read as psi * scal[SC_SCALE] (pending renormalisation)
+    int shift;                // calcnu multiplies by exp(scal[SC_SHIFT]) too (mixed imaginary time)
 };
 
 struct SolveAt {

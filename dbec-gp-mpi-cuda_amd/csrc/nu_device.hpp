@@ -19,6 +19,7 @@ struct TrapConst {
 struct NuConst {
     double dt, g, gd, nd;  // nd = Nx*Ny*Nz
     double inv_nd;         // 1/nd, used when nd is a power of two (then phi*inv_nd == phi/nd exactly)
+    double shift;          // imaginary time: exponent shift (0 = the reference's factor)
     int nd_pow2;
 };
 
@@ -33,12 +34,15 @@ __device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2
     return tc.half_par * (tc.g2 * x2 + tc.n2 * y2 + tc.l2 * z2);
 }
 
-// imaginary time: psi * exp(-dt (pot + g psi^2 + gd phi / N))
+// imaginary time: psi * exp(-dt (pot + g psi^2 + gd phi / N)), times
+// exp(shift): a constant factor that the renormalisation after the sweeps
+// removes again (the sweeps are linear); the mixed form uses it to keep FP32
+// psi away from overflow, the FP64 form passes 0 (exact: x + 0 = x).
 __device__ __forceinline__ double nu_imag(double psitmp, double phi, double pot, const NuConst &nc) {
     const double psi2lin = psitmp * psitmp * nc.g;
     const double psidd2lin = div_nd(phi, nc) * nc.gd;
     const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
-    return psitmp * exp(-pottmp);
+    return psitmp * exp(-pottmp + nc.shift);
 }
 
 // real time: psi * e^{-i dt (pot + g |psi|^2 + gd phi / N)}

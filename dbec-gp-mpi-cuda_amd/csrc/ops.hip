@@ -79,6 +79,7 @@ NuConst nu_const(const dbec_ctx *c) {
     nc.nd = (double)nd;
     nc.inv_nd = 1.0 / (double)nd;
     nc.nd_pow2 = (nd & (nd - 1)) == 0;
+    nc.shift = 0.0;
     return nc;
 }
 
@@ -130,6 +131,13 @@ hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale) {
         if (c->prec == DBEC_MIXED) k_scale<float><<<grid, block, 0, s>>>(static_cast<float *>(c->b.psi), scale, 2 * c->N);
         else k_scale<double><<<grid, block, 0, s>>>(static_cast<double *>(c->b.psi), scale, 2 * c->N);
     }
+    return hipGetLastError();
+}
+
+__global__ void k_set_scalar(double *p, double v) { *p = v; }
+
+hipError_t launch_set_scalar(hipStream_t s, double *p, double v) {
+    k_set_scalar<<<1, 1, 0, s>>>(p, v);
     return hipGetLastError();
 }
 
@@ -360,6 +368,81 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
     }
 }
 
+// Persistent form of the x / y sweep: a workgroup stages the coefficients
+// once and walks tiles t = blockIdx.x, + gridDim.x, ...; the segment loads of
+// its next tile (addresses clamped into the grid, no branches) are issued
+// before the current tile's solve, so they are in flight during the
+// recurrences and barriers (barriers wait for LDS traffic only).
+template <typename S, int AXIS, int L, int M>
+__global__ void __launch_bounds__(512)
+k_sweep_xy_p(Solves sv, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
+             const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
+             SweepConst<typename Arith<S>::T> k, int ntiles) {
+    using A = typename Arith<S>::T;
+    extern __shared__ char smem_raw[];
+    S *__restrict__ psi = reinterpret_cast<S *>(solve_at(sv).psi);
+    const int n = AXIS == 0 ? Nx : Ny;
+    const int P = n / M;
+    const int nt = L * P;
+    SweepShared<A> sh(smem_raw, n, nt, L);
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int zt = (Nz + L - 1) / L;
+    const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
+    const int e0 = s * M;
+    const int jl = e0 > 0 ? e0 - 1 : 0, jr = e0 + M < n ? e0 + M : n - 1;
+    const A zero = m_zero(k.A);
+    for (int i = tid; i < n - 1; i += nt) {
+        sh.ca[i] = calpha[i];
+        sh.cg[i] = cgamma[i];
+        sh.ga[i] = ga[i];
+    }
+    auto tile = [&](int t, bool &act) {
+        const int pl = t / zt, z0 = (t % zt) * L;
+        const int nl = min(L, Nz - z0);
+        act = l < nl;
+        return (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + (act ? l : nl - 1);
+    };
+    bool act;
+    long base = tile(blockIdx.x, act);
+    S cur[M], cl, cr;
+#pragma unroll
+    for (int j = 0; j < M; ++j) cur[j] = psi[base + (long)(e0 + j) * es];
+    cl = psi[base + (long)jl * es];
+    cr = psi[base + (long)jr * es];
+    __syncthreads();
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const bool more = t + (int)gridDim.x < ntiles;  // uniform over the workgroup
+        bool actn = false;
+        long basen = 0;
+        S nxt[M], nl_, nr_;
+        if (more) {
+            basen = tile(t + gridDim.x, actn);
+#pragma unroll
+            for (int j = 0; j < M; ++j) nxt[j] = psi[basen + (long)(e0 + j) * es];
+            nl_ = psi[basen + (long)jl * es];
+            nr_ = psi[basen + (long)jr * es];
+        }
+        A v[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) v[j] = act ? to_a(cur[j]) : zero;
+        const A hl = (act && e0 > 0) ? to_a(cl) : zero;
+        const A hr = (act && e0 + M < n) ? to_a(cr) : zero;
+        seg_solve<A, M, false>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
+        }
+        __syncthreads();  // the carry exchange is rewritten by the next tile
+#pragma unroll
+        for (int j = 0; j < M; ++j) cur[j] = nxt[j];
+        cl = nl_;
+        cr = nr_;
+        base = basen;
+        act = actn;
+    }
+}
+
 // z sweep: lines are contiguous rows, so a workgroup of L rows is staged
 // through an LDS tile in the storage type (row-major, one element of padding:
 // coalesced row loads and stores, conflict-free segment reads), each thread
@@ -479,10 +562,11 @@ static int env_int(const char *name, int dflt) {
 template <typename S>
 static int z_rows(long n) {
     static const int max_threads = env_int("DBEC_SWEEP_Z_THREADS", 256);
+    static const long tile_max = 1024L * env_int("DBEC_SWEEP_Z_TILE_KB", 48);
     const int M = z_seg_len(n);
     const int P = (int)(n / M);
     int L = 64;
-    while (L > 4 && ((long)L * P > max_threads || (long)(n + 1) * L * (long)sizeof(S) > 49152)) L /= 2;
+    while (L > 4 && ((long)L * P > max_threads || (long)(n + 1) * L * (long)sizeof(S) > tile_max)) L /= 2;
     return L;
 }
 
@@ -502,11 +586,39 @@ struct Launch {
     Solves v;
 };
 
+// Persistent x / y sweeps (DBEC_SWEEP_XY_PERSIST = workgroups per CU, 0 = off)
+template <typename S, int AXIS>
+static hipError_t sweep_xy_persist(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                   const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+                                   const SweepConst<typename Arith<S>::T> &k, int per_cu) {
+    using A = typename Arith<S>::T;
+    constexpr int M = 16;
+    constexpr int L = sizeof(S) <= 8 ? 32 : 16;
+    const int n = AXIS == 0 ? Nx : Ny;
+    const int P = n / M;
+    if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
+    const int ntiles = ((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (ncu <= 0) ncu = 256;
+    }
+    const int nblk = std::min(ntiles, ncu * per_cu);
+    dim3 grid((unsigned)nblk, (unsigned)q.v.nb), block(L * P);
+    const size_t lds = SweepShared<A>::bytes(n, L * P, L);
+    k_sweep_xy_p<S, AXIS, L, M><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k, ntiles);
+    return hipGetLastError();
+}
+
 template <typename S, int AXIS>
 static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
                                     const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
                                     const SweepConst<typename Arith<S>::T> &k) {
     using A = typename Arith<S>::T;
+    static const int per_cu = env_int("DBEC_SWEEP_XY_PERSIST", 0);
+    if (per_cu > 0) return sweep_xy_persist<S, AXIS>(q, Nx, Ny, Nz, ca, cg, ga, k, per_cu);
     constexpr int L = 256 / sizeof(S);  // 256-byte rows of z per load
     const int n = AXIS == 0 ? Nx : Ny;
     int M = seg_len<A>(n);
@@ -535,6 +647,11 @@ static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const 
     const long nblocks = (long)((Ny + L - 1) / L) * Nx;
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
     const size_t lds = ((SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15)) + (size_t)(Nz + 1) * L * sizeof(S);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_z<S, L, 16, COLS>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     k_sweep_z<S, L, 16, COLS><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
     return hipGetLastError();
 }
@@ -614,9 +731,19 @@ k_norm_cols(Solves v, int tiles, const double *__restrict__ wx, int Nx) {
         __syncthreads();
     }
     if (tid == 0) {
+        // With the shift c, psi carried an extra factor exp(c): the reference's
+        // (pre-rescale) norm is norm * exp(-c), and c - log(norm) brings the
+        // next step's norm to about 1 (c follows dt * mu).
         const double norm = sqrt(sh[0]);
-        scal[SC_NORM] = norm;
+        if (v.shift) {
+            const double c = scal[SC_SHIFT];
+            scal[SC_NORM] = norm * exp(-c);
+            scal[SC_SHIFT] = c - log(norm);
+        } else {
+            scal[SC_NORM] = norm;
+        }
         scal[SC_SCALE] = 1.0 / norm;
+        scal[SC_LASTNORM] = scal[SC_NORM];
     }
 }
 

@@ -20,6 +20,8 @@ struct SweepConst {
 hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny, long Nz);
 hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale);
 hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale);
+// *p = v on the stream (a one-thread kernel: also valid under graph capture)
+hipError_t launch_set_scalar(hipStream_t s, double *p, double v);
 // Sweep along `axis` of the solves v.
 // The grid, coefficients and weights come from c.  The z sweep with cols also
 // writes, per solve and (y tile t, x), rows[t][x] = sum over the tile's rows

@@ -20,6 +20,21 @@ def test_header_matches_binding():
     assert declared() == sorted(dbecpy.SYMBOLS)
 
 
+def declared_serve():
+    hdr = open(os.path.join(REPO, "include", "dbec_serve.h")).read()
+    return sorted(set(re.findall(r"\b(dbec_server_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_serve_header_matches_binding_and_library():
+    import dbecserve
+    assert declared_serve() == sorted(dbecserve.SERVE_SYMBOLS)
+    if not os.path.exists(dbecpy.HIP_LIB):
+        pytest.fail("libdbec_hip.so is not built")
+    lib = ctypes.CDLL(dbecpy.HIP_LIB, mode=ctypes.RTLD_LOCAL)
+    for name in declared_serve():
+        assert hasattr(lib, name), name
+
+
 @pytest.mark.parametrize("which", ["product", "oracle"])
 def test_library_exports_every_symbol(which, oracle_lib):
     path = dbecpy.HIP_LIB if which == "product" else os.path.join(REPO, "oracle", "liboracle.so")
