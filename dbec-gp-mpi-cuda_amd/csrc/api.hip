@@ -421,12 +421,17 @@ int setup(dbec_ctx *c) {
     CHK(dev_alloc(c, (void **)&b.rows, (size_t)2 * Nx * Ny * sizeof(double), "row sums"));
     CHK(dev_alloc(c, (void **)&b.scal, SC_COUNT * sizeof(double), "scalars"));
     CHK(dev_alloc(c, (void **)&b.coef, coef.size() * sizeof(double), "CN coefficients"));
+    std::vector<float> coef_f(coef.begin(), coef.end());  // FP32 tables of the mixed form's sweeps
+    if (f32) CHK(dev_alloc(c, &b.coef_lo, coef_f.size() * sizeof(float), "CN coefficients (FP32)"));
     CHK(dev_alloc(c, (void **)&b.wgt, wgt.size() * sizeof(double), "Simpson weights"));
     CHK(dev_alloc(c, (void **)&b.sq, sq.size() * sizeof(double), "grid squares"));
     hipStream_t s = c->stream;
     HIPCHK(c, hipMemsetAsync(b.psi, 0, (size_t)c->N * psi_elem(c), s), "memset");
     HIPCHK(c, hipMemsetAsync(b.phi, 0, (size_t)c->N * re, s), "memset");
     HIPCHK(c, hipMemcpyAsync(b.coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
+    if (f32)
+        HIPCHK(c, hipMemcpyAsync(b.coef_lo, coef_f.data(), coef_f.size() * sizeof(float), hipMemcpyHostToDevice, s),
+               "copy");
     HIPCHK(c, hipMemcpyAsync(b.wgt, wgt.data(), wgt.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
     HIPCHK(c, hipMemcpyAsync(b.sq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
     double sc[SC_COUNT] = {0};

@@ -355,6 +355,8 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     nc.g = sr.p.g;
     nc.gd = sr.p.gd;
     nc.shift = v.shift ? sr.scal[SC_SHIFT] : 0.0;
+    nc.cap = v.shift;
+    nc.capped = sr.scal + SC_CAPPED;
     const int tid = threadIdx.x;
     const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);

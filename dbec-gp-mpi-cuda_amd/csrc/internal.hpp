@@ -42,9 +42,10 @@ struct Buffers {
 
 // Device scalar slots.
 // SC_SHIFT: exponent shift of the mixed-precision imaginary-time step (see
-// k_norm_cols), 0 elsewhere.
+// k_norm_cols), 0 elsewhere; SC_CAPPED: set to 1 when that step had to cap
+// the growth factor (a collapsing state).
 enum { SC_NORM = 0, SC_SCALE = 1, SC_ONE = 2, SC_MU = 3, SC_EN = 4, SC_R0 = 5, SC_RX = 6, SC_RY = 7, SC_RZ = 8,
-       SC_SHIFT = 9, SC_LASTNORM = 10, SC_COUNT = 16 };
+       SC_SHIFT = 9, SC_LASTNORM = 10, SC_CAPPED = 11, SC_COUNT = 16 };
 
 struct KernelTimer;
 

@@ -20,7 +20,9 @@ struct NuConst {
     double dt, g, gd, nd;  // nd = Nx*Ny*Nz
     double inv_nd;         // 1/nd, used when nd is a power of two (then phi*inv_nd == phi/nd exactly)
     double shift;          // imaginary time: exponent shift (0 = the reference's factor)
+    double *capped;        // set to 1 where the exponent was capped
     int nd_pow2;
+    int cap;               // cap the exponent (mixed form)
 };
 
 __device__ __forceinline__ double div_nd(double phi, const NuConst &nc) {
@@ -42,7 +44,15 @@ __device__ __forceinline__ double nu_imag(double psitmp, double phi, double pot,
     const double psi2lin = psitmp * psitmp * nc.g;
     const double psidd2lin = div_nd(phi, nc) * nc.gd;
     const double pottmp = nc.dt * (pot + psi2lin + psidd2lin);
-    return psitmp * exp(-pottmp + nc.shift);
+    double e = -pottmp + nc.shift;
+    // FP32 storage: cap the growth factor at e^80 so psi stays finite, and
+    // flag it (FP64: no cap).  Only a collapsing state gets there: the
+    // shift follows dt * mu, so e stays O(1) on a converging state.
+    if (nc.cap && e > 80.0) {
+        e = 80.0;
+        *nc.capped = 1.0;
+    }
+    return psitmp * exp(e);
 }
 
 // real time: psi * e^{-i dt (pot + g |psi|^2 + gd phi / N)}

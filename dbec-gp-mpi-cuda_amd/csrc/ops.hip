@@ -80,6 +80,8 @@ NuConst nu_const(const dbec_ctx *c) {
     nc.inv_nd = 1.0 / (double)nd;
     nc.nd_pow2 = (nd & (nd - 1)) == 0;
     nc.shift = 0.0;
+    nc.cap = 0;
+    nc.capped = nullptr;
     return nc;
 }
 
@@ -177,23 +179,34 @@ __device__ __forceinline__ cpx<double> m_fma(cpx<double> a, cpx<double> b, cpx<d
 __device__ __forceinline__ cpx<double> m_zero(cpx<double>) { return {0.0, 0.0}; }
 __device__ __forceinline__ cpx<double> m_one(cpx<double>) { return {1.0, 0.0}; }
 __device__ __forceinline__ double m_mod2(cpx<double> a) { return a.x * a.x + a.y * a.y; }
+// FP32 (mixed form): the recurrences in single precision, the Simpson
+// partials of |psi|^2 in double.
+__device__ __forceinline__ float m_mul(float a, float b) { return a * b; }
+__device__ __forceinline__ float m_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float m_sub(float a, float b) { return a - b; }
+__device__ __forceinline__ float m_fma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ float m_zero(float) { return 0.0f; }
+__device__ __forceinline__ float m_one(float) { return 1.0f; }
+__device__ __forceinline__ double m_mod2(float a) { return (double)a * (double)a; }
+__device__ __forceinline__ cpx<float> m_mul(cpx<float> a, cpx<float> b) { return cmul(a, b); }
+__device__ __forceinline__ cpx<float> m_add(cpx<float> a, cpx<float> b) { return a + b; }
+__device__ __forceinline__ cpx<float> m_sub(cpx<float> a, cpx<float> b) { return a - b; }
+__device__ __forceinline__ cpx<float> m_fma(cpx<float> a, cpx<float> b, cpx<float> c) {
+    float re = fmaf(a.x, b.x, c.x);
+    re = fmaf(-a.y, b.y, re);
+    float im = fmaf(a.x, b.y, c.y);
+    im = fmaf(a.y, b.x, im);
+    return {re, im};
+}
+__device__ __forceinline__ cpx<float> m_zero(cpx<float>) { return {0.0f, 0.0f}; }
+__device__ __forceinline__ cpx<float> m_one(cpx<float>) { return {1.0f, 0.0f}; }
+__device__ __forceinline__ double m_mod2(cpx<float> a) { return (double)a.x * a.x + (double)a.y * a.y; }
 
-// Storage <-> arithmetic conversions (FP32 storage computes in FP64 inside the line).
-template <typename S> struct Arith;
-template <> struct Arith<double> { using T = double; };
-template <> struct Arith<float> { using T = double; };
-template <> struct Arith<cpx<double>> { using T = cpx<double>; };
-template <> struct Arith<cpx<float>> { using T = cpx<double>; };
-__device__ __forceinline__ double to_a(double v) { return v; }
-__device__ __forceinline__ double to_a(float v) { return (double)v; }
-__device__ __forceinline__ cpx<double> to_a(cpx<double> v) { return v; }
-__device__ __forceinline__ cpx<double> to_a(cpx<float> v) { return {(double)v.x, (double)v.y}; }
-template <typename S> __device__ __forceinline__ S from_a(double v);
-template <> __device__ __forceinline__ double from_a<double>(double v) { return v; }
-template <> __device__ __forceinline__ float from_a<float>(double v) { return (float)v; }
-template <typename S> __device__ __forceinline__ S from_a(cpx<double> v);
-template <> __device__ __forceinline__ cpx<double> from_a<cpx<double>>(cpx<double> v) { return v; }
-template <> __device__ __forceinline__ cpx<float> from_a<cpx<float>>(cpx<double> v) { return {(float)v.x, (float)v.y}; }
+// Storage and arithmetic types: FP64 lines in FP64, FP32 lines (mixed form)
+// in FP32 with FP32 coefficient tables rounded from the FP64 ones.
+template <typename S> struct Arith { using T = S; };
+template <typename S> __device__ __forceinline__ S to_a(S v) { return v; }
+template <typename S> __device__ __forceinline__ S from_a(S v) { return v; }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -368,81 +381,6 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
     }
 }
 
-// Persistent form of the x / y sweep: a workgroup stages the coefficients
-// once and walks tiles t = blockIdx.x, + gridDim.x, ...; the segment loads of
-// its next tile (addresses clamped into the grid, no branches) are issued
-// before the current tile's solve, so they are in flight during the
-// recurrences and barriers (barriers wait for LDS traffic only).
-template <typename S, int AXIS, int L, int M>
-__global__ void __launch_bounds__(512)
-k_sweep_xy_p(Solves sv, int Nx, int Ny, int Nz, const typename Arith<S>::T *__restrict__ calpha,
-             const typename Arith<S>::T *__restrict__ cgamma, const typename Arith<S>::T *__restrict__ ga,
-             SweepConst<typename Arith<S>::T> k, int ntiles) {
-    using A = typename Arith<S>::T;
-    extern __shared__ char smem_raw[];
-    S *__restrict__ psi = reinterpret_cast<S *>(solve_at(sv).psi);
-    const int n = AXIS == 0 ? Nx : Ny;
-    const int P = n / M;
-    const int nt = L * P;
-    SweepShared<A> sh(smem_raw, n, nt, L);
-    const int tid = threadIdx.x;
-    const int l = tid % L, s = tid / L;
-    const int zt = (Nz + L - 1) / L;
-    const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
-    const int e0 = s * M;
-    const int jl = e0 > 0 ? e0 - 1 : 0, jr = e0 + M < n ? e0 + M : n - 1;
-    const A zero = m_zero(k.A);
-    for (int i = tid; i < n - 1; i += nt) {
-        sh.ca[i] = calpha[i];
-        sh.cg[i] = cgamma[i];
-        sh.ga[i] = ga[i];
-    }
-    auto tile = [&](int t, bool &act) {
-        const int pl = t / zt, z0 = (t % zt) * L;
-        const int nl = min(L, Nz - z0);
-        act = l < nl;
-        return (AXIS == 0 ? (long)pl * Nz : (long)pl * Ny * Nz) + z0 + (act ? l : nl - 1);
-    };
-    bool act;
-    long base = tile(blockIdx.x, act);
-    S cur[M], cl, cr;
-#pragma unroll
-    for (int j = 0; j < M; ++j) cur[j] = psi[base + (long)(e0 + j) * es];
-    cl = psi[base + (long)jl * es];
-    cr = psi[base + (long)jr * es];
-    __syncthreads();
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const bool more = t + (int)gridDim.x < ntiles;  // uniform over the workgroup
-        bool actn = false;
-        long basen = 0;
-        S nxt[M], nl_, nr_;
-        if (more) {
-            basen = tile(t + gridDim.x, actn);
-#pragma unroll
-            for (int j = 0; j < M; ++j) nxt[j] = psi[basen + (long)(e0 + j) * es];
-            nl_ = psi[basen + (long)jl * es];
-            nr_ = psi[basen + (long)jr * es];
-        }
-        A v[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) v[j] = act ? to_a(cur[j]) : zero;
-        const A hl = (act && e0 > 0) ? to_a(cl) : zero;
-        const A hr = (act && e0 + M < n) ? to_a(cr) : zero;
-        seg_solve<A, M, false>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
-        if (act) {
-#pragma unroll
-            for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
-        }
-        __syncthreads();  // the carry exchange is rewritten by the next tile
-#pragma unroll
-        for (int j = 0; j < M; ++j) cur[j] = nxt[j];
-        cl = nl_;
-        cr = nr_;
-        base = basen;
-        act = actn;
-    }
-}
-
 // z sweep: lines are contiguous rows, so a workgroup of L rows is staged
 // through an LDS tile in the storage type (row-major, one element of padding:
 // coalesced row loads and stores, conflict-free segment reads), each thread
@@ -545,13 +483,13 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
 // into at least 4 segments, else 16 (fewer registers for complex lines).
 template <typename A>
 static int seg_len(long n) {
-    if (sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 4) return 32;
+    if (sizeof(A) <= 8 && n % 32 == 0 && n / 32 >= 4) return 32;
     return 16;
 }
 
 // z sweep: 16-element segments (more threads per tile row); rows per
-// workgroup: a power of two >= 4 whose tile fits 48 KiB of LDS, with at most
-// 512 threads.
+// workgroup: a power of two >= 4 whose tile fits 64 KiB of LDS, with at most
+// 256 threads (measured at 256^3 and 256x256x384).
 static int z_seg_len(long) { return 16; }
 
 static int env_int(const char *name, int dflt) {
@@ -562,7 +500,7 @@ static int env_int(const char *name, int dflt) {
 template <typename S>
 static int z_rows(long n) {
     static const int max_threads = env_int("DBEC_SWEEP_Z_THREADS", 256);
-    static const long tile_max = 1024L * env_int("DBEC_SWEEP_Z_TILE_KB", 48);
+    static const long tile_max = 1024L * env_int("DBEC_SWEEP_Z_TILE_KB", 64);
     const int M = z_seg_len(n);
     const int P = (int)(n / M);
     int L = 64;
@@ -586,56 +524,47 @@ struct Launch {
     Solves v;
 };
 
-// Persistent x / y sweeps (DBEC_SWEEP_XY_PERSIST = workgroups per CU, 0 = off)
-template <typename S, int AXIS>
-static hipError_t sweep_xy_persist(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                   const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
-                                   const SweepConst<typename Arith<S>::T> &k, int per_cu) {
+template <typename S, int AXIS, int L>
+static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                  const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+                                  const SweepConst<typename Arith<S>::T> &k, int M) {
     using A = typename Arith<S>::T;
-    constexpr int M = 16;
-    constexpr int L = sizeof(S) <= 8 ? 32 : 16;
     const int n = AXIS == 0 ? Nx : Ny;
-    const int P = n / M;
-    if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
-    const int ntiles = ((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
-    static int ncu = 0;
-    if (!ncu) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        if (ncu <= 0) ncu = 256;
-    }
-    const int nblk = std::min(ntiles, ncu * per_cu);
-    dim3 grid((unsigned)nblk, (unsigned)q.v.nb), block(L * P);
-    const size_t lds = SweepShared<A>::bytes(n, L * P, L);
-    k_sweep_xy_p<S, AXIS, L, M><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k, ntiles);
-    return hipGetLastError();
-}
-
-template <typename S, int AXIS>
-static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                    const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
-                                    const SweepConst<typename Arith<S>::T> &k) {
-    using A = typename Arith<S>::T;
-    static const int per_cu = env_int("DBEC_SWEEP_XY_PERSIST", 0);
-    if (per_cu > 0) return sweep_xy_persist<S, AXIS>(q, Nx, Ny, Nz, ca, cg, ga, k, per_cu);
-    constexpr int L = 256 / sizeof(S);  // 256-byte rows of z per load
-    const int n = AXIS == 0 ? Nx : Ny;
-    int M = seg_len<A>(n);
     if (L * (n / M) > 512) M = 32;
     const int P = n / M;
     if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
     const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
     const size_t lds = SweepShared<A>::bytes(n, L * P, L);
-    if constexpr (sizeof(A) == 8) {
+    if constexpr (sizeof(A) <= 8) {
         if (M == 32) {
             k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
             return hipGetLastError();
         }
+    } else {
+        if (M != 16) return hipErrorInvalidValue;
     }
     k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
     return hipGetLastError();
+}
+
+// x / y sweeps: rows of 256 bytes of z per load, segments of 32 elements
+// where the line allows four of them and the value type is at most 8 bytes,
+// else 16; FP32 real lines use 128-byte rows and 16-element segments.
+// DBEC_SWEEP_XY_{ROWB,M}[32] override (tuning runs).
+template <typename S, int AXIS>
+static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
+                                    const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+                                    const SweepConst<typename Arith<S>::T> &k) {
+    using A = typename Arith<S>::T;
+    // FP32 real lines: 128-byte rows and 16-element segments (measured at 256^3)
+    static const int rowb = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_ROWB32", 128) : env_int("DBEC_SWEEP_XY_ROWB", 256);
+    static const int mset = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_M32", 16) : env_int("DBEC_SWEEP_XY_M", 0);
+    const int n = AXIS == 0 ? Nx : Ny;
+    int M = (mset == 16 || mset == 32) ? mset : seg_len<A>(n);
+    if (sizeof(A) > 8) M = 16;
+    if (rowb == 128) return sweep_xy_launch<S, AXIS, 128 / sizeof(S)>(q, Nx, Ny, Nz, ca, cg, ga, k, M);
+    return sweep_xy_launch<S, AXIS, 256 / sizeof(S)>(q, Nx, Ny, Nz, ca, cg, ga, k, M);
 }
 
 template <typename S, int L, bool COLS>
@@ -688,26 +617,39 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves
     const long tot = (c->Nx - 1) + (c->Ny - 1) + (c->Nz - 1);
     const bool f32 = c->prec == DBEC_MIXED;
     if (c->mode == DBEC_IMAG) {
-        const double *ca = b.coef + off[axis];
-        const double *cg = b.coef + tot + off[axis];
-        const double *ga = b.coef + 2 * tot + off[axis];
+        if (f32) {
+            const float *cf = static_cast<const float *>(b.coef_lo);
+            SweepConst<float> k;
+            k.A0r = (float)c->A0r[axis][0];
+            k.A = (float)c->A[axis][0];
+            k.mA = -k.A;
+            return sweep_axis<float>(q, c, axis, cf + off[axis], cf + tot + off[axis], cf + 2 * tot + off[axis], k,
+                                     cols);
+        }
+        const double *cd = b.coef;
         SweepConst<double> k;
         k.A0r = c->A0r[axis][0];
         k.A = c->A[axis][0];
         k.mA = -c->A[axis][0];
-        return f32 ? sweep_axis<float>(q, c, axis, ca, cg, ga, k, cols)
-                   : sweep_axis<double>(q, c, axis, ca, cg, ga, k, cols);
+        return sweep_axis<double>(q, c, axis, cd + off[axis], cd + tot + off[axis], cd + 2 * tot + off[axis], k,
+                                  cols);
+    }
+    if (f32) {
+        const cpx<float> *cf = static_cast<const cpx<float> *>(b.coef_lo);
+        SweepConst<cpx<float>> k;
+        k.A0r = {(float)c->A0r[axis][0], (float)c->A0r[axis][1]};
+        k.A = {(float)c->A[axis][0], (float)c->A[axis][1]};
+        k.mA = {(float)c->mA[axis][0], (float)c->mA[axis][1]};
+        return sweep_axis<cpx<float>>(q, c, axis, cf + off[axis], cf + tot + off[axis], cf + 2 * tot + off[axis], k,
+                                      cols);
     }
     const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
-    const cpx<double> *ca = cc + off[axis];
-    const cpx<double> *cg = cc + tot + off[axis];
-    const cpx<double> *ga = cc + 2 * tot + off[axis];
     SweepConst<cpx<double>> k;
     k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
     k.A = {c->A[axis][0], c->A[axis][1]};
     k.mA = {c->mA[axis][0], c->mA[axis][1]};
-    return f32 ? sweep_axis<cpx<float>>(q, c, axis, ca, cg, ga, k, cols)
-               : sweep_axis<cpx<double>>(q, c, axis, ca, cg, ga, k, cols);
+    return sweep_axis<cpx<double>>(q, c, axis, cc + off[axis], cc + tot + off[axis], cc + 2 * tot + off[axis], k,
+                                   cols);
 }
 
 // norm^2 = sum_x wx[x] sum_t colpart[t][x]; scal[NORM] = norm, scal[SCALE] = 1/norm.

@@ -40,6 +40,11 @@ __device__ __forceinline__ void put_psi(R *psi, long n, double v, bool cplx) {
     }
 }
 
+__global__ void k_widen(double *__restrict__ dst, const float *__restrict__ src, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (double)src[t];
+}
+
 template <typename R>
 __global__ void k_gauss(R *__restrict__ psi, const double *__restrict__ sq, int Nx, int Ny, int Nz, double vg,
                         double vn, double vl, double cpsi, int cplx) {
@@ -60,9 +65,11 @@ size_t round_up(size_t b) { return (b + 255) & ~size_t(255); }
 struct Job {
     dbec_request r;
     double t_submit = 0., t_start = 0.;
-    long steps = 0;
+    long steps = 0, steps_fp64 = 0;
     double mu_prev = 0.;
     bool have_prev = false;
+    bool polishing = false;  // moved to an FP64 slot
+    bool to_polish = false;  // mixed phase done, waiting for an FP64 slot
 };
 
 struct Bucket {
@@ -260,9 +267,9 @@ struct dbec_server {
         return true;
     }
 
-    int admit(Bucket *b, int s, std::unique_ptr<Job> j) {
+    // trap and couplings of the solve in slot s
+    void set_params(Bucket *b, int s, const dbec_request &r) {
         dbec_ctx *v = b->views[s];
-        const dbec_request &r = j->r;
         v->p.vgamma = r.vgamma;
         v->p.vnu = r.vnu;
         v->p.vlambda = r.vlambda;
@@ -271,6 +278,51 @@ struct dbec_server {
         b->h_par[s].tc = trap_const(v);
         b->h_par[s].g = r.g;
         b->h_par[s].gd = r.gd;
+    }
+
+    Bucket *find_bucket(const dbec_request &r) {
+        for (auto &k : buckets)
+            if (same_bucket(k->key, r)) return k.get();
+        return nullptr;
+    }
+
+    // A mixed ground state that reached polish_tol continues in FP64: its
+    // (renormalised) psi is widened into a free slot of the FP64 bucket of
+    // the same grid, on that bucket's stream; the mixed bucket's chunk has
+    // completed, so the source is final.
+    int migrate(Bucket *from, int s) {
+        dbec_request key = from->job[s]->r;
+        key.prec = DBEC_FP64;
+        Bucket *b = find_bucket(key);
+        if (!b) {
+            if (!make_room(bucket_bytes(key, cfg.max_batch))) return DBEC_OK;  // retry later
+            int rc = make_bucket(key, &b);
+            if (rc != DBEC_OK) return rc;
+        }
+        const int d = b->inflight ? -1 : b->free_slot();
+        if (d < 0) return DBEC_OK;
+        dbec_ctx *v = b->views[d];
+        set_params(b, d, key);
+        const long n = v->N;
+        k_widen<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(static_cast<double *>(v->b.psi),
+                                                                     static_cast<const float *>(from->views[s]->b.psi), n);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SHIFT, 0.0);
+        if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_CAPPED, 0.0);
+        if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SCALE, 1.0);
+        if (e != hipSuccess) return fail(std::string("polish: ") + hipGetErrorString(e));
+        std::unique_ptr<Job> j = std::move(from->job[s]);
+        j->polishing = true;
+        j->to_polish = false;
+        j->have_prev = false;
+        b->job[d] = std::move(j);
+        return DBEC_OK;
+    }
+
+    int admit(Bucket *b, int s, std::unique_ptr<Job> j) {
+        dbec_ctx *v = b->views[s];
+        const dbec_request &r = j->r;
+        set_params(b, s, r);
         const double pi = 3.14159265358979;
         const double cpsi = std::sqrt(pi * std::sqrt(pi / (r.vgamma * r.vnu * r.vlambda)));
         const dim3 blk(256), grd((unsigned)((v->N + 255) / 256));
@@ -283,6 +335,7 @@ struct dbec_server {
                                                          (int)v->Ny, (int)v->Nz, r.vgamma, r.vnu, r.vlambda, cpsi, cplx);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SHIFT, 0.0);
+        if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_CAPPED, 0.0);
         if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SCALE, 1.0);
         if (e == hipSuccess && v->mode == DBEC_IMAG) {
             // imag3d normalises psi0 (imag3d.cu:239): the first step reads psi / norm
@@ -302,13 +355,14 @@ struct dbec_server {
         b->launched.clear();
         long K = cfg.check_every;
         for (int s = 0; s < b->cap; s++) {
-            if (!b->job[s]) continue;
+            if (!b->job[s] || b->job[s]->to_polish) continue;
             b->launched.push_back(s);
             if (!imag) K = std::min(K, b->job[s]->r.max_steps - b->job[s]->steps);
             else K = std::min(K, std::max(1L, b->job[s]->r.max_steps - b->job[s]->steps));
         }
         const int na = (int)b->launched.size();
         if (na == 0) return DBEC_OK;
+        const bool fp64 = t->prec == DBEC_FP64;
         for (int i = 0; i < na; i++) b->h_slot[i] = b->launched[i];
         hipStream_t st = t->stream;
         hipError_t e = hipMemcpyAsync(b->d_slot, b->h_slot, na * sizeof(int), hipMemcpyHostToDevice, st);
@@ -337,6 +391,7 @@ struct dbec_server {
         for (int s : b->launched) {
             Job &j = *b->job[s];
             j.steps += K;
+            if (fp64 && j.polishing) j.steps_fp64 += K;
             int what = 0;
             if (imag) what = OBS_MUEN | OBS_RMS;
             else if (j.steps >= j.r.max_steps) what = OBS_MUEN | OBS_RMS;
@@ -362,13 +417,16 @@ struct dbec_server {
             int status = -1;
             if (imag) {
                 const double mu = sc[SC_MU];
-                if (!std::isfinite(mu)) status = DBEC_SOLVE_FAILED;
-                else if (j.have_prev && std::fabs(mu - j.mu_prev) <= j.r.mutol * std::fabs(mu)) status = DBEC_SOLVE_CONVERGED;
+                const double dmu = std::fabs(mu - j.mu_prev);
+                const bool polish = b->tab->prec == DBEC_MIXED && cfg.polish_tol > j.r.mutol;
+                if (!std::isfinite(mu) || sc[SC_CAPPED] != 0.) status = DBEC_SOLVE_DIVERGED;
+                else if (polish && j.have_prev && dmu <= cfg.polish_tol * std::fabs(mu)) j.to_polish = true;
+                else if (j.have_prev && dmu <= j.r.mutol * std::fabs(mu)) status = DBEC_SOLVE_CONVERGED;
                 else if (j.steps >= j.r.max_steps) status = DBEC_SOLVE_NOT_CONVERGED;
                 j.mu_prev = mu;
                 j.have_prev = true;
             } else if (j.steps >= j.r.max_steps) {
-                status = std::isfinite(sc[SC_MU]) ? DBEC_SOLVE_STEPS_DONE : DBEC_SOLVE_FAILED;
+                status = std::isfinite(sc[SC_MU]) ? DBEC_SOLVE_STEPS_DONE : DBEC_SOLVE_DIVERGED;
             }
             if (status < 0) continue;
             dbec_result res;
@@ -376,6 +434,7 @@ struct dbec_server {
             res.id = j.r.id;
             res.status = status;
             res.steps = j.steps;
+            res.steps_fp64 = j.steps_fp64;
             res.norm = imag ? sc[SC_LASTNORM] : sc[SC_NORM];
             res.mu = sc[SC_MU];
             res.en = sc[SC_EN];
@@ -400,6 +459,17 @@ struct dbec_server {
             if (q != hipSuccess) return fail(std::string("chunk: ") + hipGetErrorString(q));
             retire(b.get());
             changed = true;
+        }
+        // mixed ground states that wait for an FP64 slot
+        for (size_t bi = 0; bi < buckets.size(); bi++) {
+            Bucket *b = buckets[bi].get();
+            if (b->inflight) continue;
+            for (int s = 0; s < b->cap; s++) {
+                if (!b->job[s] || !b->job[s]->to_polish) continue;
+                int rc = migrate(b, s);
+                if (rc != DBEC_OK) return rc;
+                if (!b->job[s]) changed = true;
+            }
         }
         // admission, in submission order
         for (auto it = queue.begin(); it != queue.end();) {

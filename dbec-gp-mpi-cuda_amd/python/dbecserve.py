@@ -22,12 +22,12 @@ SERVE_SYMBOLS = [
     "dbec_server_progress", "dbec_server_pending", "dbec_server_usage", "dbec_server_last_error",
 ]
 
-CONVERGED, STEPS_DONE, NOT_CONVERGED, FAILED = range(4)
+CONVERGED, STEPS_DONE, NOT_CONVERGED, DIVERGED = range(4)
 
 
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("mem_budget", ctypes.c_double), ("max_batch", ctypes.c_int),
-                ("check_every", ctypes.c_long)]
+                ("check_every", ctypes.c_long), ("polish_tol", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):
@@ -44,14 +44,15 @@ class Request(ctypes.Structure):
 
 class Result(ctypes.Structure):
     _fields_ = [
-        ("id", ctypes.c_long), ("status", ctypes.c_int), ("steps", ctypes.c_long),
+        ("id", ctypes.c_long), ("status", ctypes.c_int), ("steps", ctypes.c_long), ("steps_fp64", ctypes.c_long),
         ("norm", ctypes.c_double), ("mu", ctypes.c_double), ("en", ctypes.c_double),
         ("rms", ctypes.c_double * 4),
         ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
     ]
 
     def as_dict(self):
-        return {"id": self.id, "status": self.status, "steps": self.steps, "norm": self.norm, "mu": self.mu,
+        return {"id": self.id, "status": self.status, "steps": self.steps, "steps_fp64": self.steps_fp64,
+                "norm": self.norm, "mu": self.mu,
                 "en": self.en, "rms": list(self.rms), "t_submit": self.t_submit, "t_start": self.t_start,
                 "t_done": self.t_done}
 
@@ -85,10 +86,11 @@ def make_request(rid, n, mode, prec, g0, gd0, trap, dt, max_steps, mutol=1e-8, c
 
 
 class Server:
-    def __init__(self, lib, device=0, mem_budget=0.0, max_batch=16, check_every=50):
+    def __init__(self, lib, device=0, mem_budget=0.0, max_batch=16, check_every=50, polish_tol=1e-6):
         self.lib = declare(lib)
         self.h = ctypes.c_void_p()
-        cfg = Config(device=device, mem_budget=mem_budget, max_batch=max_batch, check_every=check_every)
+        cfg = Config(device=device, mem_budget=mem_budget, max_batch=max_batch, check_every=check_every,
+                     polish_tol=polish_tol)
         rc = self.lib.dbec_server_create(ctypes.byref(cfg), ctypes.byref(self.h))
         if rc != 0:
             raise dbecpy.DbecError(rc, "server creation failed")
@@ -204,7 +206,7 @@ def plain_solve(lib, req, check_every=50):
             if req.mode == dbecpy.IMAG:
                 mu, en = s.calcmuen()
                 if not np.isfinite(mu):
-                    status = FAILED
+                    status = DIVERGED
                 elif mu_prev is not None and abs(mu - mu_prev) <= req.mutol * abs(mu):
                     status = CONVERGED
                 elif steps >= req.max_steps:

@@ -39,6 +39,9 @@ typedef struct {
     double mem_budget;   /* bytes of device memory the server may hold; <= 0: half the free memory */
     int max_batch;       /* slots per bucket (solves per launch), 1..64 */
     long check_every;    /* steps per chunk: mu check, admission and retirement points */
+    double polish_tol;   /* > 0: a DBEC_MIXED ground state that reaches |dmu/mu| <= polish_tol moves
+                            to an FP64 slot of the same grid and converges there to its mutol
+                            (FP32 cannot resolve mu to 1e-8); <= 0: mixed solves run to mutol */
 } dbec_server_config;
 
 typedef struct {
@@ -53,13 +56,17 @@ typedef struct {
     double mutol;        /* imaginary time: stop rule on |dmu/mu| between checks */
 } dbec_request;
 
-/* Status of a result. */
-enum { DBEC_SOLVE_CONVERGED = 0, DBEC_SOLVE_STEPS_DONE = 1, DBEC_SOLVE_NOT_CONVERGED = 2, DBEC_SOLVE_FAILED = 3 };
+/* Status of a result.  DIVERGED: psi blew up (NaN or overflow), or the
+ * mixed-precision step had to cap its growth factor; both happen when the
+ * requested condensate collapses (strong dipolar interaction), and FP64 and
+ * mixed solves are stopped at the first check after it. */
+enum { DBEC_SOLVE_CONVERGED = 0, DBEC_SOLVE_STEPS_DONE = 1, DBEC_SOLVE_NOT_CONVERGED = 2, DBEC_SOLVE_DIVERGED = 3 };
 
 typedef struct {
     long id;
     int status;
-    long steps;          /* split steps taken */
+    long steps;          /* split steps taken (mixed and FP64 polish together) */
+    long steps_fp64;     /* of which in the FP64 polish */
     double norm;         /* norm of psi (imaginary time: the last step's pre-rescale norm) */
     double mu, en;       /* raw calcmuen values (the programs print them divided by par) */
     double rms[4];       /* <r>, <x>, <y>, <z> */
