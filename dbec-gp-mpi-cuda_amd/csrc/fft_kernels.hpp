@@ -61,12 +61,10 @@ struct PsiTraits<cpx<double>> {
         return a * a;
     }
 };
+// mixed form: |psi|^2 of complex64 psi in FP32 (the spectrum is FP32 too)
 template <>
 struct PsiTraits<cpx<float>> {
-    static __device__ __forceinline__ double mod2(cpx<float> v, double) {
-        double a = hypot((double)v.x, (double)v.y);
-        return a * a;
-    }
+    static __device__ __forceinline__ double mod2(cpx<float> v, double) { return v.x * v.x + v.y * v.y; }
 };
 
 // ---------------------------------------------------------------------------

@@ -71,15 +71,28 @@ __device__ __forceinline__ cpx<double> nu_real(cpx<double> v, double phi, double
 __device__ __forceinline__ double nu_apply(double v, double s, double phi, double pot, const NuConst &nc) {
     return nu_imag(v * s, phi, pot, nc);
 }
-__device__ __forceinline__ float nu_apply(float v, double s, double phi, double pot, const NuConst &nc) {
-    return (float)nu_imag((double)v * s, phi, pot, nc);
-}
 __device__ __forceinline__ cpx<double> nu_apply(cpx<double> v, double, double phi, double pot, const NuConst &nc) {
     return nu_real(v, phi, pot, nc);
 }
+// The mixed form evaluates the nonlinear factor in FP32 (single-precision
+// exp / sincos), with the renormalisation, the exponent shift and phi / N
+// formed in FP64 first.
+__device__ __forceinline__ float nu_apply(float v, double s, double phi, double pot, const NuConst &nc) {
+    const float p = (float)((double)v * s);
+    const float lin = (float)pot + p * p * (float)nc.g + (float)(div_nd(phi, nc) * nc.gd);
+    float e = -(float)nc.dt * lin + (float)nc.shift;
+    if (nc.cap && e > 80.0f) {
+        e = 80.0f;
+        *nc.capped = 1.0;
+    }
+    return p * expf(e);
+}
 __device__ __forceinline__ cpx<float> nu_apply(cpx<float> v, double, double phi, double pot, const NuConst &nc) {
-    cpx<double> r = nu_real(cpx<double>{(double)v.x, (double)v.y}, phi, pot, nc);
-    return {(float)r.x, (float)r.y};
+    const float a2 = v.x * v.x + v.y * v.y;
+    const float lin = (float)pot + a2 * (float)nc.g + (float)(div_nd(phi, nc) * nc.gd);
+    float sn, cs;
+    sincosf(-(float)nc.dt * lin, &sn, &cs);
+    return {v.x * cs - v.y * sn, v.x * sn + v.y * cs};
 }
 
 }  // namespace dbec

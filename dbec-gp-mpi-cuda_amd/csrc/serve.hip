@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -87,6 +88,10 @@ struct Bucket {
     double *h_scal = nullptr;
     hipEvent_t ev = nullptr;
     bool inflight = false;
+    // chunk graphs: check_every steps of nb solves, by nb (the slot list and
+    // parameters are device data the graph reads, so any composition of nb
+    // solves replays the same graph)
+    std::vector<hipGraphExec_t> graphs;
     std::vector<int> launched;
     double bytes = 0.;
     double last_used = 0.;
@@ -182,6 +187,8 @@ struct dbec_server {
         (void)hipHostFree(b->h_par);
         (void)hipHostFree(b->h_scal);
         if (b->ev) (void)hipEventDestroy(b->ev);
+        for (auto g : b->graphs)
+            if (g) (void)hipGraphExecDestroy(g);
         (void)dbec_destroy(b->tab);
     }
 
@@ -384,9 +391,32 @@ struct dbec_server {
         v.nb = na;
         v.use_scale = imag ? 1 : 0;
         v.shift = (imag && t->prec == DBEC_MIXED) ? 1 : 0;
-        for (long k = 0; k < K; k++) {
-            int rc = core_step(t, v);
-            if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+        static const bool use_graphs = [] {
+            const char *e = getenv("DBEC_SERVE_GRAPHS");
+            return !e || atoi(e) != 0;
+        }();
+        if (use_graphs && K == cfg.check_every) {
+            if ((int)b->graphs.size() <= na) b->graphs.resize(na + 1, nullptr);
+            if (!b->graphs[na]) {
+                // the step kernels only launch: no allocation, copy or memset
+                hipGraph_t g = nullptr;
+                e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+                int rc = DBEC_OK;
+                for (long k = 0; k < K && e == hipSuccess && rc == DBEC_OK; k++) rc = core_step(t, v);
+                hipError_t e2 = hipStreamEndCapture(st, &g);
+                if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+                if (e == hipSuccess) e = e2;
+                if (e == hipSuccess) e = hipGraphInstantiate(&b->graphs[na], g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(std::string("chunk graph: ") + hipGetErrorString(e));
+            }
+            e = hipGraphLaunch(b->graphs[na], st);
+            if (e != hipSuccess) return fail(std::string("chunk graph launch: ") + hipGetErrorString(e));
+        } else {
+            for (long k = 0; k < K; k++) {
+                int rc = core_step(t, v);
+                if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+            }
         }
         for (int s : b->launched) {
             Job &j = *b->job[s];
