@@ -140,6 +140,7 @@ Solves solve_ref(const dbec_ctx *c, bool use_scale) {
     v.nb = 1;
     v.use_scale = use_scale ? 1 : 0;
     v.shift = (c->prec == DBEC_MIXED && c->mode == DBEC_IMAG) ? 1 : 0;
+    v.fold = 0;
     return v;
 }
 
@@ -158,6 +159,8 @@ FftArgs fft_args(const dbec_ctx *c, int axis, const Solves &v) {
     a.tw = a.fp32 ? c->b.twf[axis] : c->b.tw[axis];
     a.sq = c->b.sq;
     a.nc = nu_const(c);
+    a.wx = c->b.wgt;
+    a.tiles = (int)sweep_z_tiles(c);
     return a;
 }
 
@@ -207,17 +210,22 @@ int run_sweep(dbec_ctx *c, int axis, const Solves &v, bool cols) {
     return DBEC_OK;
 }
 
-// One split step (calcpsidd2, calcnu, calclux/y/z and, in imaginary time,
-// the norm whose 1/norm scale the next read applies) of the solves v.
+// One split step (calcpsidd2, calcnu, calclux/y/z) of the solves v.  In
+// imaginary time the z sweep leaves the Simpson column partials of the new
+// psi: the next step folds the norm in (v.fold), or run_norm computes it.
 int run_step(dbec_ctx *c, const Solves &v) {
     const bool imag = c->mode == DBEC_IMAG;
     CHK(run_conv_nu(c, v));
     CHK(run_sweep(c, 0, v, false));
     CHK(run_sweep(c, 1, v, false));
     CHK(run_sweep(c, 2, v, imag));
-    if (imag)
-        HIPCHK(c, timed(c, "norm", v.nb * (double)c->Nx * sweep_z_tiles(c) * 8.0,
-                        [&] { return launch_norm_from_cols(c->stream, c, v); }), "norm");
+    return DBEC_OK;
+}
+
+// calcnorm from the column partials: scal[NORM], scal[SCALE] = 1/norm.
+int run_norm(dbec_ctx *c, const Solves &v) {
+    HIPCHK(c, timed(c, "norm", v.nb * (double)c->Nx * sweep_z_tiles(c) * 8.0,
+                    [&] { return launch_norm_from_cols(c->stream, c, v); }), "norm");
     return DBEC_OK;
 }
 
@@ -702,8 +710,14 @@ int dbec_step(dbec_ctx *c, long nsteps, double g_stp, double gd_stp, double *nor
             c->g += g_stp;
             c->gd += gd_stp;
         }
-        CHK(run_step(c, solve_ref(c, c->scale_pending)));
-        c->scale_pending = imag;
+        // after the first step the previous norm is folded into this one
+        Solves v = solve_ref(c, imag && (c->scale_pending || s > 0));
+        v.fold = (imag && s > 0) ? 1 : 0;
+        CHK(run_step(c, v));
+    }
+    if (imag && nsteps > 0) {
+        CHK(run_norm(c, solve_ref(c, false)));
+        c->scale_pending = true;
     }
     if (imag && nsteps > 0 && norm) {
         CHK(materialize_scale(c));
@@ -900,6 +914,8 @@ namespace dbec {
 Solves core_solves(const dbec_ctx *c, bool use_scale) { return solve_ref(c, use_scale); }
 
 int core_step(dbec_ctx *c, const Solves &v) { return run_step(c, v); }
+
+int core_norm(dbec_ctx *c, const Solves &v) { return run_norm(c, v); }
 
 int core_observe_async(dbec_ctx *c, int what) {
     if (c->mode == DBEC_IMAG) {

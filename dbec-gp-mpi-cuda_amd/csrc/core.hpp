@@ -11,8 +11,11 @@ namespace dbec {
 
 // The single solve held by c (its own buffers, trap and couplings).
 Solves core_solves(const dbec_ctx *c, bool use_scale);
-// One split step of the solves v; c supplies grid, tables and stream.
+// One split step of the solves v; c supplies grid, tables and stream.  In
+// imaginary time the step leaves the norm in the column partials: set v.fold
+// on the next step, or call core_norm.
 int core_step(dbec_ctx *c, const Solves &v);
+int core_norm(dbec_ctx *c, const Solves &v);
 // Observables of c's psi, asynchronous on c->stream, results in c->b.scal.
 // Imaginary time first applies the pending 1/norm (scal[SC_SCALE] -> psi,
 // then scal[SC_SCALE] = 1).  what & OBS_MUEN: mu, E; what & OBS_RMS: norm and

@@ -19,6 +19,7 @@
 #include <cstdlib>
 
 #include "internal.hpp"
+#include "norm_device.hpp"
 
 namespace dbec {
 
@@ -70,7 +71,7 @@ struct PsiTraits<cpx<float>> {
 // ---------------------------------------------------------------------------
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
+k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *__restrict__ wx, int tiles) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
@@ -82,7 +83,7 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
     const int x = blockIdx.x / ytiles;
     const int y0 = (blockIdx.x % ytiles) * 2 * L;
     const int nrows = min(2 * L, Ny - y0);
-    const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
+    const double s = (v.use_scale && !v.fold) ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
     // Rows 2p and 2p+1 of the tile become the real and imaginary parts of
     // line p.  A thread takes VE consecutive z of both rows, so its LDS
@@ -132,6 +133,10 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
             spec[((long)k * Nx + x) * Ny + y0 + row] = o;
         }
     }
+    if (v.fold && blockIdx.x == 0) {  // the last step's norm, for the x convolution and calcnu
+        __syncthreads();
+        norm_from_cols<WG>(sr.rows, tiles, wx, Nx, sr.scal, v.shift, reinterpret_cast<double *>(buf));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -178,7 +183,10 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
-    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(solve_at(v).spec);
+    const SolveAt sr = solve_at(v);
+    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(sr.spec);
+    // folded norm: |psi|^2 was transformed unscaled, so potdd takes 1/norm^2
+    const R s2 = v.fold ? (R)(sr.scal[SC_SCALE] * sr.scal[SC_SCALE]) : R(1);
     const int tid = threadIdx.x;
     const int ytiles = (Ny + L - 1) / L;
     const int kz = blockIdx.x / ytiles;
@@ -221,7 +229,7 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
         const int i = tid + q * WG;
         if (i < NXH * L) {
             const int ll = i % L, kx = i / L;
-            const R w = pw[q];
+            const R w = pw[q] * s2;
             cpx<R> v = buf[kx * LS + ll];
             buf[kx * LS + ll] = {v.x * w, v.y * w};
             if (kx > 0 && kx < N / 2) {
@@ -479,12 +487,12 @@ struct FftLaunch {
         const int Nx = (int)a.Nx, Ny = (int)a.Ny;
         if (!a.fp32) {
             auto *tw = static_cast<const cpx<double> *>(a.tw);
-            if (a.cpsi) k_zfwd<N, double, cpx<double>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
-            else k_zfwd<N, double, double, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
+            if (a.cpsi) k_zfwd<N, double, cpx<double>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw, a.wx, a.tiles);
+            else k_zfwd<N, double, double, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw, a.wx, a.tiles);
         } else {
             auto *tw = static_cast<const cpx<float> *>(a.tw);
-            if (a.cpsi) k_zfwd<N, float, cpx<float>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
-            else k_zfwd<N, float, float, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw);
+            if (a.cpsi) k_zfwd<N, float, cpx<float>, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw, a.wx, a.tiles);
+            else k_zfwd<N, float, float, LM><<<grid, block, 0, a.stream>>>(a.v, Nx, Ny, tw, a.wx, a.tiles);
         }
         return hipGetLastError();
     }

@@ -99,6 +99,9 @@ struct Solves {
     int nb;                   // launch rows (gridDim.y)
     int use_scale;            // psi is read as psi * scal[SC_SCALE] (pending renormalisation)
     int shift;                // calcnu multiplies by exp(scal[SC_SHIFT]) too (mixed imaginary time)
+    int fold;                 // the last step's norm is still in the column partials: the forward z
+                              // pass computes it (workgroup 0 of each solve), reads psi unscaled, and
+                              // the x convolution applies 1/norm^2 (the transform is linear)
 };
 
 struct SolveAt {
@@ -131,6 +134,8 @@ struct FftArgs {
     const void *tw;       // twiddles of the transformed axis (FP64 or FP32)
     const double *sq;     // x^2 | y^2 | z^2 (fused calcnu)
     NuConst nc;           // dt and grid size; couplings come from the solve
+    const double *wx;     // Simpson weights along x (folded norm)
+    int tiles;            // rows of the column partials (folded norm)
 };
 // Each returns hipSuccess or an error; `n` is the transformed length.
 hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C along z)

@@ -5,6 +5,7 @@
 // src/utils/diffint.cu.
 #include <cstdlib>
 
+#include "norm_device.hpp"
 #include "ops.hpp"
 
 namespace dbec {
@@ -657,36 +658,7 @@ __global__ void __launch_bounds__(256)
 k_norm_cols(Solves v, int tiles, const double *__restrict__ wx, int Nx) {
     __shared__ double sh[256];
     const SolveAt sr = solve_at(v);
-    const double *__restrict__ colpart = sr.rows;
-    double *__restrict__ scal = sr.scal;
-    const int tid = threadIdx.x;
-    double p = 0.0;
-    for (int x = tid; x < Nx; x += 256) {
-        double cs = 0.0;
-        for (int t = 0; t < tiles; t++) cs += colpart[(long)t * Nx + x];
-        p += wx[x] * cs;
-    }
-    sh[tid] = p;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) sh[tid] += sh[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        // With the shift c, psi carried an extra factor exp(c): the reference's
-        // (pre-rescale) norm is norm * exp(-c), and c - log(norm) brings the
-        // next step's norm to about 1 (c follows dt * mu).
-        const double norm = sqrt(sh[0]);
-        if (v.shift) {
-            const double c = scal[SC_SHIFT];
-            scal[SC_NORM] = norm * exp(-c);
-            scal[SC_SHIFT] = c - log(norm);
-        } else {
-            scal[SC_NORM] = norm;
-        }
-        scal[SC_SCALE] = 1.0 / norm;
-        scal[SC_LASTNORM] = scal[SC_NORM];
-    }
+    norm_from_cols<256>(sr.rows, tiles, wx, Nx, sr.scal, v.shift, sh);
 }
 
 hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const Solves &v) {

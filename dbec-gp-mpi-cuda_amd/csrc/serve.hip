@@ -391,6 +391,9 @@ struct dbec_server {
         v.nb = na;
         v.use_scale = imag ? 1 : 0;
         v.shift = (imag && t->prec == DBEC_MIXED) ? 1 : 0;
+        v.fold = 0;
+        Solves vf = v;  // steps after the first fold the previous norm in
+        vf.fold = imag ? 1 : 0;
         static const bool use_graphs = [] {
             const char *e = getenv("DBEC_SERVE_GRAPHS");
             return !e || atoi(e) != 0;
@@ -402,7 +405,8 @@ struct dbec_server {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
                 int rc = DBEC_OK;
-                for (long k = 0; k < K && e == hipSuccess && rc == DBEC_OK; k++) rc = core_step(t, v);
+                for (long k = 0; k < K && e == hipSuccess && rc == DBEC_OK; k++) rc = core_step(t, k ? vf : v);
+                if (imag && rc == DBEC_OK) rc = core_norm(t, v);
                 hipError_t e2 = hipStreamEndCapture(st, &g);
                 if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
                 if (e == hipSuccess) e = e2;
@@ -414,8 +418,12 @@ struct dbec_server {
             if (e != hipSuccess) return fail(std::string("chunk graph launch: ") + hipGetErrorString(e));
         } else {
             for (long k = 0; k < K; k++) {
-                int rc = core_step(t, v);
+                int rc = core_step(t, k ? vf : v);
                 if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+            }
+            if (imag) {
+                int rc = core_norm(t, v);
+                if (rc != DBEC_OK) return fail(std::string("norm: ") + t->err);
             }
         }
         for (int s : b->launched) {
