@@ -397,6 +397,11 @@ int setup(dbec_ctx *c) {
                 pq[(k * qx + i) * qy + j] = (4. * pi * (3. * kz2[k] / (kx2[i] + ky2[j] + kz2[k]) - 1.) / 3.) * tmp;
             }
     pq[0] = 0.;
+    // device layout [kz][|ky|][|kx|]: rows contiguous in kx for the x convolution
+    std::vector<double> pqd(pq.size());
+    for (long k = 0; k < qz; k++)
+        for (long i = 0; i < qx; i++)
+            for (long j = 0; j < qy; j++) pqd[(k * qy + j) * qx + i] = pq[(k * qx + i) * qy + j];
     // host copy of the half table [kx][ky][kz<=Nz/2] for dbec_get_table
     c->h_potdd_half.resize(Nx * Ny * qz);
     for (long i = 0; i < Nx; i++)
@@ -448,9 +453,9 @@ int setup(dbec_ctx *c) {
     sc[SC_ONE] = 1.0;
     HIPCHK(c, hipMemcpyAsync(b.scal, sc, sizeof sc, hipMemcpyHostToDevice, s), "copy");
     if (!f32) {
-        HIPCHK(c, hipMemcpyAsync(b.potddq, pq.data(), pq.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
+        HIPCHK(c, hipMemcpyAsync(b.potddq, pqd.data(), pqd.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
     } else {
-        std::vector<float> pf(pq.begin(), pq.end());
+        std::vector<float> pf(pqd.begin(), pqd.end());
         HIPCHK(c, hipMemcpyAsync(b.potddq, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice, s), "copy");
         HIPCHK(c, hipStreamSynchronize(s), "synchronize");
     }

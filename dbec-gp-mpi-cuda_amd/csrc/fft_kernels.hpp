@@ -178,7 +178,7 @@ k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg) {
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     constexpr int NXH = N / 2 + 1;
     constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
@@ -193,18 +193,19 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     const int y0 = (blockIdx.x % ytiles) * L;
     const int nl = min(L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
-    // potdd(|kx|, |ky|, kz) for kx <= N/2, fetched into registers up front;
-    // each value multiplies both kx and N - kx
+    // potdd(|kx|, |ky|, kz) for kx <= N/2 (table rows [kz][|ky|] contiguous
+    // in kx, lanes along kx), fetched into registers up front; each value
+    // multiplies both kx and N - kx
     const int Ny2 = Ny / 2, Nyh = Ny2 + 1;
     const R *pk = potddq + (long)kz * NXH * Nyh;
     R pw[QP];
 #pragma unroll
     for (int q = 0; q < QP; ++q) {
         const int i = tid + q * WG;
-        const int ll = i % L, kx = i / L;
+        const int kx = i % NXH, ll = i / NXH;
         const int y = y0 + ll;
         const int ky = y <= Ny2 ? y : Ny - y;
-        pw[q] = (i < NXH * L && ll < nl) ? pk[kx * Nyh + ky] : R(0);
+        pw[q] = (i < NXH * L && ll < nl) ? pk[ky * NXH + kx] : R(0);
     }
     cpx<R> *g = spec + (long)kz * N * Ny + y0;
     constexpr int K = L * N / WG;
@@ -228,7 +229,7 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     for (int q = 0; q < QP; ++q) {
         const int i = tid + q * WG;
         if (i < NXH * L) {
-            const int ll = i % L, kx = i / L;
+            const int kx = i % NXH, ll = i / NXH;
             const R w = pw[q] * s2;
             cpx<R> v = buf[kx * LS + ll];
             buf[kx * LS + ll] = {v.x * w, v.y * w};

@@ -20,7 +20,7 @@ namespace dbec {
 //                           out so that y lines are contiguous and x lines are
 //                           strided by Ny (coalesced over y tiles)
 //   phi    [x][y][z]        real: unnormalised dipolar potential (psidd2)
-//   potddq [kz][kx][ky]     real, kz <= Nz/2, kx <= Nx/2, ky <= Ny/2: the
+//   potddq [kz][ky][kx]     real, kz <= Nz/2, kx <= Nx/2, ky <= Ny/2: the
 //                           dipolar kernel is even in every k component, so a
 //                           quarter table indexed by |k| holds it exactly
 //   rows   [2][Ny][Nx]      double: per-row z integrals for the reductions

@@ -162,6 +162,48 @@ def test_imag_256_short(hip_lib, oracle_lib):
         assert g.calcmuen() == pytest.approx(o.calcmuen(), rel=TOL)
 
 
+def test_real_256x256x384_short(hip_lib, oracle_lib):
+    """The real-time workload (256x256x384, dx = 0.1, dt = 2e-4, gd quenched to
+    45 from a Gaussian), 3 steps: psi, norm and rms against the oracle."""
+    say("real 256x256x384 x 3")
+    p = dbecpy.make_params(256, 256, 384, 0.1, 0.1, 0.1, 0.0002, 0.5, 1.0, 1.5, 12.8)
+    psi0 = dbecpy.gaussian_psi0(p).astype(np.complex128)
+    with dbecpy.Solver(hip_lib, p, dbecpy.REAL) as g, dbecpy.Solver(oracle_lib, p, dbecpy.REAL) as o:
+        for s in (g, o):
+            s.set_psi(psi0)
+            s.set_coupling(2 * 40.0, 2 * 45.0)
+            s.step(3, want_norm=False)
+        assert rel(g.get_psi(), o.get_psi()) < TOL
+        assert g.calcnorm() == pytest.approx(o.calcnorm(), rel=TOL)
+        assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
+
+
+def test_imag_512_fp64_and_mixed(hip_lib, oracle_lib):
+    """512^3 (dx = 0.05, dt = 1.25e-4, R = 12.8): two FP64 steps against the
+    oracle, and 100 steps of the mixed form against FP64 (mu to 1e-6)."""
+    say("imag 512^3: FP64 x 2 vs oracle, mixed x 100 vs FP64")
+    p = dbecpy.make_params(512, 512, 512, 0.05, 0.05, 0.05, 0.000125, 0.5, 1.0, 1.5, 12.8)
+    psi0 = dbecpy.gaussian_psi0(p)
+    with dbecpy.Solver(hip_lib, p) as g, dbecpy.Solver(oracle_lib, p) as o:
+        for s in (g, o):
+            s.set_psi(psi0)
+            s.calcnorm(rescale=True)
+            s.set_coupling(80.0, 60.0)
+        assert g.step(2) == pytest.approx(o.step(2), rel=TOL)
+        assert rel(g.get_psi(), o.get_psi()) < TOL
+    del psi0
+    with dbecpy.Solver(hip_lib, p) as d, dbecpy.Solver(hip_lib, p, dbecpy.IMAG, dbecpy.MIXED) as m:
+        psi0 = dbecpy.gaussian_psi0(p)
+        for s in (d, m):
+            s.set_psi(psi0)
+            s.calcnorm(rescale=True)
+            s.set_coupling(80.0, 60.0)
+            s.step(100)
+        mud, mum = d.calcmuen()[0], m.calcmuen()[0]
+        say(f"  mu fp64 {mud:.12g} mixed {mum:.12g} rel {abs(mum - mud) / abs(mud):.2e}")
+        assert abs(mum - mud) / abs(mud) < 1e-6
+
+
 def test_reference_chain_product_programs(tmp_path):
     """The product's imag3d -> real3d chain on the reference's example inputs
     against the reference's shipped outputs."""
