@@ -340,8 +340,8 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 // phi[.][.][0] of the same row, so the row's potential never leaves the chip.
 // psi of 8 bytes or less per element is fetched into registers before the
 // transform, so its latency overlaps the spectrum load; complex FP64 psi is
-// fetched after it in batches of 8.  The squared coordinates of the trap
-// potential are staged in LDS.
+// fetched after it in batches of 8.  The per-row and per-z parts of the trap
+// potential (trap_xy, trap_z) are staged in LDS.
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, int nfaceb,
@@ -353,8 +353,8 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     static_assert((2 * L * N) % WG == 0 && Q % C == 0, "zinv_nu tile");
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
-    __shared__ double sqz[N];
-    __shared__ double sqr[2 * L];  // interior: y^2 of the tile's rows; faces: x^2 | y^2 of each face row
+    __shared__ double sqz[N];      // trap_z of each z
+    __shared__ double sqr[2 * L];  // trap_xy of the tile's rows (faces: of each face row)
     const SolveAt sr = solve_at(v);
     const cpx<R> *__restrict__ spec = reinterpret_cast<const cpx<R> *>(sr.spec);
     P *__restrict__ psi = reinterpret_cast<P *>(sr.psi);
@@ -364,10 +364,11 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     nc.shift = v.shift ? sr.scal[SC_SHIFT] : 0.0;
     nc.cap = v.shift;
     nc.capped = sr.scal + SC_CAPPED;
+    nu_prepare(nc);
     const int tid = threadIdx.x;
     const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
     stage_twiddles(tw, twg, N, tid, WG);
-    for (int i = tid; i < N; i += WG) sqz[i] = sq[Nx + Ny + i];
+    for (int i = tid; i < N; i += WG) sqz[i] = trap_z(tc, sq[Nx + Ny + i]);
     if ((int)blockIdx.x >= nfaceb) {
         const int b = (int)blockIdx.x - nfaceb;
         const int ytiles = (Ny + 2 * L - 1) / (2 * L);
@@ -375,7 +376,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         const int y0 = (b % ytiles) * 2 * L;
         const int nrows = min(2 * L, Ny - y0);
         P *dst = psi + ((long)x * Ny + y0) * N;
-        if (tid < 2 * L) sqr[tid] = sq[Nx + min(y0 + tid, Ny - 1)];
+        if (tid < 2 * L) sqr[tid] = trap_xy(tc, sq[x], sq[Nx + min(y0 + tid, Ny - 1)]);
         P pv[C];
         if constexpr (PREFETCH) {
 #pragma unroll
@@ -391,7 +392,6 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         });
         __syncthreads();
         FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
-        const double x2 = sq[x];
 #pragma unroll
         for (int q0 = 0; q0 < Q; q0 += C) {
             if constexpr (!PREFETCH) {
@@ -407,8 +407,8 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
                 const int z = i % N, row = i / N;
                 if (row < nrows && y0 + row < Ny - 1) {
                     const cpx<R> cv = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
-                    const double phi = (double)((row & 1) ? cv.y : cv.x);
-                    const double pot = trap(tc, x2, sqr[row], sqz[z]);
+                    const R phi = (row & 1) ? cv.y : cv.x;
+                    const double pot = trap_sum(tc, sqr[row], sqz[z]);
                     dst[(long)row * N + z] = nu_apply(pv[PREFETCH ? q0 + c : c], s, phi, pot, nc);
                 }
             }
@@ -421,8 +421,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     const int nl = min(L, nface - f0);
     if (tid < L) {
         const int f = f0 + min(tid, nl - 1);
-        sqr[tid] = sq[f < Ny ? Nx - 1 : f - Ny];
-        sqr[L + tid] = sq[Nx + (f < Ny ? f : Ny - 1)];
+        sqr[tid] = trap_xy(tc, sq[f < Ny ? Nx - 1 : f - Ny], sq[Nx + (f < Ny ? f : Ny - 1)]);
     }
     load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
         if (p >= nl) return 0;
@@ -459,8 +458,8 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
                 const int f = f0 + p;
                 const bool from_src = (f < Ny) ? (f < Ny - 1) && (z < N - 1) : (z < N - 1);
                 const cpx<R> cv = buf[z * LS + p];
-                const double phi = (double)(from_src ? cv.y : cv.x);
-                const double pot = trap(tc, sqr[p], sqr[L + p], sqz[z]);
+                const R phi = from_src ? cv.y : cv.x;
+                const double pot = trap_sum(tc, sqr[p], sqz[z]);
                 psi[fn[c]] = nu_apply(fv[c], s, phi, pot, nc);
             }
         }

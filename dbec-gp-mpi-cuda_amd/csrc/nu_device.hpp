@@ -23,17 +23,37 @@ struct NuConst {
     double *capped;        // set to 1 where the exponent was capped
     int nd_pow2;
     int cap;               // cap the exponent (mixed form)
+    float dtf, gf, gdnf, shiftf;  // mixed form: dt, g, gd / nd, shift in FP32 (nu_prepare)
 };
+
+// Fills the FP32 constants of the mixed form from dt, g, gd, nd and shift.
+__host__ __device__ inline void nu_prepare(NuConst &nc) {
+    nc.dtf = (float)nc.dt;
+    nc.gf = (float)nc.g;
+    nc.gdnf = (float)(nc.gd / nc.nd);
+    nc.shiftf = (float)nc.shift;
+}
 
 __device__ __forceinline__ double div_nd(double phi, const NuConst &nc) {
     return nc.nd_pow2 ? phi * nc.inv_nd : phi / nc.nd;
 }
 
 // 0.5*par*(g^2 x^2 + n^2 y^2 + l^2 z^2) in the reference's evaluation order
-// with no contraction, i.e. the number the reference's pot table holds.
-__device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2, double z2) {
+// with no contraction, i.e. the number the reference's pot table holds.  It
+// splits into a per-row part g^2 x^2 + n^2 y^2, a per-z part l^2 z^2 and
+// their combination, each rounded as in the whole expression, so that
+// kernels can stage the first two and form the same number per element.
+__device__ __forceinline__ double trap_xy(const TrapConst &tc, double x2, double y2) {
 #pragma clang fp contract(off)
-    return tc.half_par * (tc.g2 * x2 + tc.n2 * y2 + tc.l2 * z2);
+    return tc.g2 * x2 + tc.n2 * y2;
+}
+__device__ __forceinline__ double trap_z(const TrapConst &tc, double z2) { return tc.l2 * z2; }
+__device__ __forceinline__ double trap_sum(const TrapConst &tc, double pxy, double pz) {
+#pragma clang fp contract(off)
+    return tc.half_par * (pxy + pz);
+}
+__device__ __forceinline__ double trap(const TrapConst &tc, double x2, double y2, double z2) {
+    return trap_sum(tc, trap_xy(tc, x2, y2), trap_z(tc, z2));
 }
 
 // imaginary time: psi * exp(-dt (pot + g psi^2 + gd phi / N)), times
@@ -75,23 +95,23 @@ __device__ __forceinline__ cpx<double> nu_apply(cpx<double> v, double, double ph
     return nu_real(v, phi, pot, nc);
 }
 // The mixed form evaluates the nonlinear factor in FP32 (single-precision
-// exp / sincos), with the renormalisation, the exponent shift and phi / N
-// formed in FP64 first.
-__device__ __forceinline__ float nu_apply(float v, double s, double phi, double pot, const NuConst &nc) {
+// exp / sincos) from FP32 phi and the FP32 constants of nu_prepare; the
+// renormalisation and the trap potential are formed in FP64 first.
+__device__ __forceinline__ float nu_apply(float v, double s, float phi, double pot, const NuConst &nc) {
     const float p = (float)((double)v * s);
-    const float lin = (float)pot + p * p * (float)nc.g + (float)(div_nd(phi, nc) * nc.gd);
-    float e = -(float)nc.dt * lin + (float)nc.shift;
+    const float lin = (float)pot + p * p * nc.gf + phi * nc.gdnf;
+    float e = -nc.dtf * lin + nc.shiftf;
     if (nc.cap && e > 80.0f) {
         e = 80.0f;
         *nc.capped = 1.0;
     }
     return p * expf(e);
 }
-__device__ __forceinline__ cpx<float> nu_apply(cpx<float> v, double, double phi, double pot, const NuConst &nc) {
+__device__ __forceinline__ cpx<float> nu_apply(cpx<float> v, double, float phi, double pot, const NuConst &nc) {
     const float a2 = v.x * v.x + v.y * v.y;
-    const float lin = (float)pot + a2 * (float)nc.g + (float)(div_nd(phi, nc) * nc.gd);
+    const float lin = (float)pot + a2 * nc.gf + phi * nc.gdnf;
     float sn, cs;
-    sincosf(-(float)nc.dt * lin, &sn, &cs);
+    sincosf(-nc.dtf * lin, &sn, &cs);
     return {v.x * cs - v.y * sn, v.x * sn + v.y * cs};
 }
 

@@ -4,6 +4,7 @@
 // real3d.cu = src/real3d-mpicuda/real3d-mpicuda.cu, diffint.cu =
 // src/utils/diffint.cu.
 #include <cstdlib>
+#include <type_traits>
 
 #include "norm_device.hpp"
 #include "ops.hpp"
@@ -59,7 +60,7 @@ k_calcnu(P *__restrict__ psi, const R *__restrict__ phi, const double *__restric
     const long n = row * Nz + k;
     const double s = scale ? *scale : 1.0;
     const double pot = trap(tc, sq[i], sq[Nx + j], sq[Nx + Ny + k]);
-    psi[n] = nu_apply(psi[n], s, (double)phi[n], pot, nc);
+    psi[n] = nu_apply(psi[n], s, phi[n], pot, nc);
 }
 
 TrapConst trap_const(const dbec_ctx *c) {
@@ -83,6 +84,7 @@ NuConst nu_const(const dbec_ctx *c) {
     nc.shift = 0.0;
     nc.cap = 0;
     nc.capped = nullptr;
+    nu_prepare(nc);
     return nc;
 }
 
@@ -265,28 +267,42 @@ __device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre
 
 // The partitioned solve of one segment (registers v, halos hl / hr).  Every
 // thread of the workgroup calls it.  Returns, with COLS, the Simpson z
-// partial sum_j wz[e] |psi'|^2 of the segment.
-template <typename A, int M, bool COLS>
+// partial sum_j wz[e] |psi'|^2 of the segment.  Of the line's elements only
+// 0 (first segment, j = 0) and n-1 (last segment, j = M-1) fall outside the
+// recurrences' range 1..n-2, so with PEEL each pass tests only its first and
+// last iteration and runs the M-2 between them unconditionally; without it
+// every iteration keeps the test.  The x / y sweeps of complex lines use the
+// latter: peeled, the compiler schedules them with more registers, which
+// measured slower there (and faster in every other sweep).
+template <typename A, int M, bool COLS, bool PEEL>
 __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n, int s, int l, int L, int P,
                                             const SweepShared<A> &sh, const SweepConst<A> &k,
                                             const double *__restrict__ wz) {
+    static_assert(M >= 2, "segment length");
+    const bool in0 = s > 0, inM = s < P - 1;  // j = 0 / j = M-1 inside 1..n-2
+    auto inr = [&](int j) { return PEEL || (e0 + j >= 1 && e0 + j <= n - 2); };  // middle iterations
     const A zero = m_zero(k.A), one = m_one(k.A);
     A bq = zero, bp = one;
     {
         A pn = hr, pc = v[M - 1];
-#pragma unroll
-        for (int j = M - 1; j >= 0; --j) {
+        auto back = [&](int j, A pp) {
             const int i = e0 + j;
-            const A pp = (j > 0) ? v[j - 1] : hl;
-            if (i >= 1 && i <= n - 2) {
-                const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-                bq = m_mul(sh.cg[i], m_sub(m_mul(k.A, bq), c));
-                bp = m_mul(sh.ga[i], bp);
-                v[j] = bq;
-            }
+            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+            bq = m_mul(sh.cg[i], m_sub(m_mul(k.A, bq), c));
+            bp = m_mul(sh.ga[i], bp);
+            v[j] = bq;
+        };
+        if (inM) back(M - 1, v[M - 2]);
+        pn = pc;
+        pc = v[M - 2];
+#pragma unroll
+        for (int j = M - 2; j >= 1; --j) {
+            const A pp = v[j - 1];
+            if (inr(j)) back(j, pp);
             pn = pc;
             pc = pp;
         }
+        if (in0) back(0, hl);
     }
     sh.exa[s * L + l] = bq;
     sh.exb[s * L + l] = bp;
@@ -296,26 +312,30 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     for (int t = P - 1; t > s; --t) x = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], x));
     {
         A pr = one;
+        auto fixb = [&](int j) {
+            pr = m_mul(sh.ga[e0 + j], pr);
+            v[j] = m_add(v[j], m_mul(pr, x));
+        };
+        if (inM) fixb(M - 1);
 #pragma unroll
-        for (int j = M - 1; j >= 0; --j) {
-            const int i = e0 + j;
-            if (i >= 1 && i <= n - 2) {
-                pr = m_mul(sh.ga[i], pr);
-                v[j] = m_add(v[j], m_mul(pr, x));
-            }
-        }
+        for (int j = M - 2; j >= 1; --j)
+            if (inr(j)) fixb(j);
+        if (in0) fixb(0);
     }
     __syncthreads();  // carries read; the exchange is reused below
     A fy = zero, fr = one;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const int i = e0 + j - 1;
-        if (i + 1 >= 1 && i + 1 <= n - 2) {
-            const A a = sh.ca[i];
+    {
+        auto fwd = [&](int j) {
+            const A a = sh.ca[e0 + j - 1];
             fy = m_fma(a, fy, v[j]);
             fr = m_mul(a, fr);
             v[j] = fy;
-        }
+        };
+        if (in0) fwd(0);
+#pragma unroll
+        for (int j = 1; j <= M - 2; ++j)
+            if (inr(j)) fwd(j);
+        if (inM) fwd(M - 1);
     }
     sh.exa[s * L + l] = fy;
     sh.exb[s * L + l] = fr;
@@ -325,15 +345,16 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     double acc = 0.0;
     {
         A pr = one;
+        auto fixf = [&](int j) {
+            pr = m_mul(sh.ca[e0 + j - 1], pr);
+            v[j] = m_add(v[j], m_mul(pr, y));
+            if (COLS) acc += wz[e0 + j] * m_mod2(v[j]);
+        };
+        if (in0) fixf(0);
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            const int i = e0 + j - 1;
-            if (i + 1 >= 1 && i + 1 <= n - 2) {
-                pr = m_mul(sh.ca[i], pr);
-                v[j] = m_add(v[j], m_mul(pr, y));
-                if (COLS) acc += wz[e0 + j] * m_mod2(v[j]);
-            }
-        }
+        for (int j = 1; j <= M - 2; ++j)
+            if (inr(j)) fixf(j);
+        if (inM) fixf(M - 1);
     }
     if (s == 0) v[0] = zero;
     if (s == P - 1) v[M - 1] = zero;
@@ -375,7 +396,7 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
     const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
     coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
     __syncthreads();
-    seg_solve<A, M, false>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
+    seg_solve<A, M, false, std::is_floating_point<A>::value>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
 #pragma unroll
         for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
@@ -444,7 +465,7 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
     for (int j = 0; j < M; ++j) v[j] = act ? to_a(row[e0 + j]) : zero;
     const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
     const A hr = (act && e0 + M < n) ? to_a(row[e0 + M]) : zero;
-    const double acc = seg_solve<A, M, COLS>(v, hl, hr, e0, n, s, l, L, P, sh, k, wz);
+    const double acc = seg_solve<A, M, COLS, true>(v, hl, hr, e0, n, s, l, L, P, sh, k, wz);
     // every thread read its segment before the first barrier inside seg_solve
     if (act) {
         S *orow = tile + l * RS;
@@ -480,11 +501,11 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
     }
 }
 
-// Segment length of a line of n elements: 32 for real FP64 lines that split
-// into at least 4 segments, else 16 (fewer registers for complex lines).
+// Segment length of a line of n elements: 32 for complex FP32 lines that
+// split into at least 4 segments, else 16 (measured at 256^3 and 192^3).
 template <typename A>
 static int seg_len(long n) {
-    if (sizeof(A) <= 8 && n % 32 == 0 && n / 32 >= 4) return 32;
+    if (!std::is_floating_point<A>::value && sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 4) return 32;
     return 16;
 }
 
@@ -549,9 +570,8 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
     return hipGetLastError();
 }
 
-// x / y sweeps: rows of 256 bytes of z per load, segments of 32 elements
-// where the line allows four of them and the value type is at most 8 bytes,
-// else 16; FP32 real lines use 128-byte rows and 16-element segments.
+// x / y sweeps: rows of 256 bytes of z per load, segments of seg_len
+// elements; FP32 real lines use 128-byte rows and 16-element segments.
 // DBEC_SWEEP_XY_{ROWB,M}[32] override (tuning runs).
 template <typename S, int AXIS>
 static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
