@@ -39,6 +39,12 @@ WORKLOADS = {
     "real192_mixed": ((192, 192, 192), 16 / 192, 0.001, 8.0, "real", "mixed"),
     "real192_fp64": ((192, 192, 192), 16 / 192, 0.001, 8.0, "real", "fp64"),
     "imag192_mixed": ((192, 192, 192), 16 / 192, 0.01, 8.0, "imag", "mixed"),
+    "imag192_fp64": ((192, 192, 192), 16 / 192, 0.01, 8.0, "imag", "fp64"),
+    "real160_mixed": ((160, 160, 160), 16 / 160, 0.001, 8.0, "real", "mixed"),
+    "imag160_mixed": ((160, 160, 160), 16 / 160, 0.01, 8.0, "imag", "mixed"),
+    "imag160_fp64": ((160, 160, 160), 16 / 160, 0.01, 8.0, "imag", "fp64"),
+    "real96_mixed": ((96, 96, 96), 16 / 96, 0.001, 8.0, "real", "mixed"),
+    "imag96_mixed": ((96, 96, 96), 16 / 96, 0.01, 8.0, "imag", "mixed"),
 }
 
 

@@ -10,14 +10,11 @@
 #include <vector>
 
 #include "core.hpp"
-#include "fft_kernels.hpp"
+#include "fft_device.hpp"
+#include "fft_launch.hpp"
 #include "ops.hpp"
 
 namespace dbec {
-
-#define DBEC_X(NN) extern template struct FftLaunch<NN>;
-DBEC_FOR_EACH_FFT_LEN(DBEC_X)
-#undef DBEC_X
 
 hipError_t launch_zfwd(long n, const FftArgs &a) {
     switch (n) {
@@ -316,65 +313,73 @@ int setup(dbec_ctx *c) {
     simpson_weights(p.dy, Ny, wgt.data() + Nx);
     simpson_weights(p.dz, Nz, wgt.data() + Nx + Ny);
 
-    // Crank-Nicolson coefficients (gencoef imag3d.cu:894-931; real3d.cu:991-1035).
+    // Crank-Nicolson coefficients (gencoef imag3d.cu:894-931; real3d.cu:991-1035)
+    // and the carry multipliers of the partitioned sweeps (CoefLayout).
     const double h2[3] = {p.dx * p.dx, p.dy * p.dy, p.dz * p.dz};
     const long n3[3] = {Nx, Ny, Nz};
-    const long tot = (Nx - 1) + (Ny - 1) + (Nz - 1);
-    // [calpha | cgamma | cgamma * A] per axis x, y, z
-    std::vector<double> coef((cplx ? 2 : 1) * 3 * tot);
-    long off = 0;
+    const CoefLayout lay(Nx, Ny, Nz);
+    const int cw = cplx ? 2 : 1;  // doubles per element
+    std::vector<double> coef(cw * lay.size(), 0.0);
+    auto put = [&](long at, Cx v) {
+        coef[cw * at] = v.re;
+        if (cplx) coef[cw * at + 1] = v.im;
+    };
     for (int a = 0; a < 3; a++) {
         const long n = n3[a];
         const double r = p.dt / h2[a] / (3. - par);
-        if (!cplx) {
-            const double A0 = 1. + r, A0r = 1. - r, A = -0.5 * p.dt / h2[a] / (3. - par);
-            c->A0r[a][0] = A0r; c->A0r[a][1] = 0.;
-            c->A[a][0] = A; c->A[a][1] = 0.;
-            c->mA[a][0] = -A; c->mA[a][1] = 0.;
-            std::vector<double> al(n - 1), ga(n - 1);
-            al[n - 2] = 0.;
-            ga[n - 2] = -1. / A0;
-            for (long i = n - 2; i > 0; i--) {
-                al[i - 1] = A * ga[i];
-                ga[i - 1] = -1. / (A0 + A * al[i - 1]);
-            }
-            for (long i = 0; i < n - 1; i++) {
-                coef[off + i] = al[i];
-                coef[tot + off + i] = ga[i];
-                coef[2 * tot + off + i] = ga[i] * A;
-            }
-            c->h_calpha[a] = al;
-            c->h_cgamma[a] = ga;
-        } else {
-            const Cx A0 = {1., r}, A0r = {1., -r}, A = {0., -0.5 * p.dt / h2[a] / (3. - par)};
-            const Cx mA = {0., -A.im}, minus1 = {-1., 0.};
-            c->A0r[a][0] = A0r.re; c->A0r[a][1] = A0r.im;
-            c->A[a][0] = A.re; c->A[a][1] = A.im;
-            c->mA[a][0] = mA.re; c->mA[a][1] = mA.im;
-            std::vector<Cx> al(n - 1), ga(n - 1);
-            al[n - 2] = {0., 0.};
-            ga[n - 2] = cx_div(minus1, A0);
-            for (long i = n - 2; i > 0; i--) {
+        // imaginary time: real A0, A0r, A; real time: A0 = 1 + i r, A0r = 1 - i r, A = -i r / 2
+        const Cx A0 = cplx ? Cx{1., r} : Cx{1. + r, 0.};
+        const Cx A0r = cplx ? Cx{1., -r} : Cx{1. - r, 0.};
+        const Cx A = cplx ? Cx{0., -0.5 * p.dt / h2[a] / (3. - par)} : Cx{-0.5 * p.dt / h2[a] / (3. - par), 0.};
+        const Cx minus1 = {-1., 0.};
+        c->A0r[a][0] = A0r.re; c->A0r[a][1] = A0r.im;
+        c->A[a][0] = A.re; c->A[a][1] = A.im;
+        c->mA[a][0] = -A.re; c->mA[a][1] = -A.im;
+        std::vector<Cx> al(n - 1), ga(n - 1);
+        al[n - 2] = {0., 0.};
+        ga[n - 2] = cplx ? cx_div(minus1, A0) : Cx{-1. / A0.re, 0.};
+        for (long i = n - 2; i > 0; i--) {
+            if (cplx) {
                 al[i - 1] = cx_mul(A, ga[i]);
                 ga[i - 1] = cx_div(minus1, cx_add(A0, cx_mul(A, al[i - 1])));
-            }
-            c->h_calpha[a].resize(2 * (n - 1));
-            c->h_cgamma[a].resize(2 * (n - 1));
-            for (long i = 0; i < n - 1; i++) {
-                coef[2 * (off + i)] = al[i].re;
-                coef[2 * (off + i) + 1] = al[i].im;
-                coef[2 * (tot + off + i)] = ga[i].re;
-                coef[2 * (tot + off + i) + 1] = ga[i].im;
-                const Cx gA = cx_mul(ga[i], A);
-                coef[2 * (2 * tot + off + i)] = gA.re;
-                coef[2 * (2 * tot + off + i) + 1] = gA.im;
-                c->h_calpha[a][2 * i] = al[i].re;
-                c->h_calpha[a][2 * i + 1] = al[i].im;
-                c->h_cgamma[a][2 * i] = ga[i].re;
-                c->h_cgamma[a][2 * i + 1] = ga[i].im;
+            } else {
+                al[i - 1] = {A.re * ga[i].re, 0.};
+                ga[i - 1] = {-1. / (A0.re + A.re * al[i - 1].re), 0.};
             }
         }
-        off += n - 1;
+        c->h_calpha[a].assign(cw * (n - 1), 0.);
+        c->h_cgamma[a].assign(cw * (n - 1), 0.);
+        for (long i = 0; i < n - 1; i++) {
+            put(lay.ca(a) + i, al[i]);
+            put(lay.cg(a) + i, ga[i]);
+            c->h_calpha[a][cw * i] = al[i].re;
+            c->h_cgamma[a][cw * i] = ga[i].re;
+            if (cplx) {
+                c->h_calpha[a][cw * i + 1] = al[i].im;
+                c->h_cgamma[a][cw * i + 1] = ga[i].im;
+            }
+        }
+        // gb[i] = prod_{t=i..top} cgamma[t] A and fa[e] = prod_{t=bot..e} calpha[t-1]
+        // within each segment of M elements, over the in-range indices 1..n-2,
+        // multiplied in the order of the sequential chains they replace
+        auto mul = [&](Cx x, Cx y) { return cplx ? cx_mul(x, y) : Cx{x.re * y.re, 0.}; };
+        for (int m = 0; m < 2; m++) {
+            const long M = 16L << m;
+            if (n % M != 0) continue;
+            for (long e0 = 0; e0 < n; e0 += M) {
+                const long bot = std::max(e0, 1L), top = std::min(e0 + M - 1, n - 2);
+                Cx pr = {1., 0.};
+                for (long i = top; i >= bot; i--) {
+                    pr = mul(mul(ga[i], A), pr);
+                    put(lay.gb(a, m) + i, pr);
+                }
+                pr = {1., 0.};
+                for (long e = bot; e <= top; e++) {
+                    pr = mul(al[e - 1], pr);
+                    put(lay.fa(a, m) + e, pr);
+                }
+            }
+        }
     }
 
     // Dipolar kernel (initpotdd imag3d.cu:945-980) on the quarter |k| grid:

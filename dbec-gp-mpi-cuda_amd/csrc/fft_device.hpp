@@ -224,6 +224,10 @@ __device__ __forceinline__ void stockham(cpx<R> *buf, int LS, int l, int t, cons
 }
 
 // Radix plan per supported length.  T = threads per line = N / (first radix).
+// First radices of 4 or 8 give T >= 12 threads per line: fewer values per
+// thread (fewer registers in the fused passes, whose psi tile is 2N / T
+// values per thread) and workgroups of whole wavefronts; at every length but
+// 384 these measured faster than plans that start with radix 16.
 template <int N>
 struct FftPlan;
 
@@ -240,17 +244,17 @@ struct FftPlan;
 
 DBEC_FFT_PLAN(16, 1, 16)
 DBEC_FFT_PLAN(32, 4, 8, 4)
-DBEC_FFT_PLAN(48, 3, 16, 3)
-DBEC_FFT_PLAN(64, 8, 8, 8)
-DBEC_FFT_PLAN(80, 5, 16, 5)
-DBEC_FFT_PLAN(96, 6, 16, 2, 3)
-DBEC_FFT_PLAN(128, 8, 16, 8)
-DBEC_FFT_PLAN(160, 10, 16, 2, 5)
-DBEC_FFT_PLAN(192, 12, 16, 4, 3)
-DBEC_FFT_PLAN(256, 16, 16, 16)
-DBEC_FFT_PLAN(320, 20, 16, 4, 5)
+DBEC_FFT_PLAN(48, 12, 4, 4, 3)
+DBEC_FFT_PLAN(64, 16, 4, 4, 4)
+DBEC_FFT_PLAN(80, 20, 4, 4, 5)
+DBEC_FFT_PLAN(96, 12, 8, 4, 3)
+DBEC_FFT_PLAN(128, 16, 8, 16)
+DBEC_FFT_PLAN(160, 20, 8, 4, 5)
+DBEC_FFT_PLAN(192, 24, 8, 8, 3)
+DBEC_FFT_PLAN(256, 32, 8, 8, 4)
+DBEC_FFT_PLAN(320, 40, 8, 8, 5)
 DBEC_FFT_PLAN(384, 24, 16, 8, 3)
-DBEC_FFT_PLAN(512, 32, 16, 16, 2)
+DBEC_FFT_PLAN(512, 64, 8, 8, 8)
 
 #undef DBEC_FFT_PLAN
 

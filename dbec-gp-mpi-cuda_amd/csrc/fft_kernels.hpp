@@ -18,6 +18,7 @@
 
 #include <cstdlib>
 
+#include "fft_launch.hpp"
 #include "internal.hpp"
 #include "norm_device.hpp"
 
@@ -468,7 +469,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
 
 // ---------------------------------------------------------------------------
 // Launchers for one length.  Each pass has a tile multiplier LM (1 or 2)
-// chosen per precision from measurements at 256^3; the environment variables
+// chosen per precision and plan from measurements; the environment variables
 // DBEC_FFT_LM_{ZFWD,YPASS,XCONV,FUSED}{64,32} override them (tuning runs).
 inline int fft_lm(const char *name, int dflt) {
     const char *e = getenv(name);
@@ -478,7 +479,14 @@ inline int fft_lm(const char *name, int dflt) {
 }
 
 template <int N>
-struct FftLaunch {
+struct FftLaunchImpl {
+    // Plans whose workgroup at LM = 1 has fewer than 192 threads take LM = 2
+    // where wide tiles measured faster (FP32 passes, FP64 fused with real
+    // psi); plans of 192 threads or more keep LM = 1 there (measured at
+    // 256^3 and 192^3).  FP32 xconv always takes LM = 2, the other FP64
+    // passes LM = 1.
+    static constexpr int WIDE = FftTile<N, 1>::WG < 192 ? 2 : 1;
+
     template <int LM>
     static hipError_t zfwd_t(const FftArgs &a) {
         using Tl = FftTile<N, LM>;
@@ -497,7 +505,7 @@ struct FftLaunch {
         return hipGetLastError();
     }
     static hipError_t zfwd(const FftArgs &a) {
-        static const int l64 = fft_lm("DBEC_FFT_LM_ZFWD64", 1), l32 = fft_lm("DBEC_FFT_LM_ZFWD32", 2);
+        static const int l64 = fft_lm("DBEC_FFT_LM_ZFWD64", 1), l32 = fft_lm("DBEC_FFT_LM_ZFWD32", WIDE);
         return (a.fp32 ? l32 : l64) == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
     }
     // FP32 spectra use tiles twice as wide, so every strided access still
@@ -520,7 +528,7 @@ struct FftLaunch {
         return hipGetLastError();
     }
     static hipError_t ypass(int dir, const FftArgs &a) {
-        static const int l64 = fft_lm("DBEC_FFT_LM_YPASS64", 1), l32 = fft_lm("DBEC_FFT_LM_YPASS32", 2);
+        static const int l64 = fft_lm("DBEC_FFT_LM_YPASS64", 1), l32 = fft_lm("DBEC_FFT_LM_YPASS32", WIDE);
         return (a.fp32 ? l32 : l64) == 2 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a);
     }
 
@@ -575,11 +583,22 @@ struct FftLaunch {
         return a.cpsi ? fused_t<cpx<float>, float, LM>(a) : fused_t<float, float, LM>(a);
     }
     static hipError_t fused(const FftArgs &a) {
-        // FP64: wide tiles for real psi, narrow ones for complex psi
-        static const int l64 = fft_lm("DBEC_FFT_LM_FUSED64", 0), l32 = fft_lm("DBEC_FFT_LM_FUSED32", 2);
-        const int lm = a.fp32 ? l32 : (l64 ? l64 : (a.cpsi ? 1 : 2));
+        // FP64: narrow tiles for complex psi
+        static const int l64 = fft_lm("DBEC_FFT_LM_FUSED64", 0), l32 = fft_lm("DBEC_FFT_LM_FUSED32", WIDE);
+        const int lm = a.fp32 ? l32 : (l64 ? l64 : (a.cpsi ? 1 : WIDE));
         return lm == 2 ? fused_lm<2>(a) : fused_lm<1>(a);
     }
 };
+
+template <int N>
+hipError_t FftLaunch<N>::zfwd(const FftArgs &a) { return FftLaunchImpl<N>::zfwd(a); }
+template <int N>
+hipError_t FftLaunch<N>::ypass(int dir, const FftArgs &a) { return FftLaunchImpl<N>::ypass(dir, a); }
+template <int N>
+hipError_t FftLaunch<N>::xconv(const FftArgs &a) { return FftLaunchImpl<N>::xconv(a); }
+template <int N>
+hipError_t FftLaunch<N>::zinv(const FftArgs &a) { return FftLaunchImpl<N>::zinv(a); }
+template <int N>
+hipError_t FftLaunch<N>::fused(const FftArgs &a) { return FftLaunchImpl<N>::fused(a); }
 
 }  // namespace dbec

@@ -1,0 +1,20 @@
+// fft_launch.hpp - declarations of the FFT pass launchers of one transform
+// length.  Their definitions and the kernels live in fft_kernels.hpp and are
+// compiled once per length (fft_inst.hip); callers include only this header,
+// so they hold no copy of any kernel or launch geometry.
+#pragma once
+
+#include "internal.hpp"
+
+namespace dbec {
+
+template <int N>
+struct FftLaunch {
+    static hipError_t zfwd(const FftArgs &a);
+    static hipError_t ypass(int dir, const FftArgs &a);
+    static hipError_t xconv(const FftArgs &a);
+    static hipError_t zinv(const FftArgs &a);   // single solve: spectrum -> phi
+    static hipError_t fused(const FftArgs &a);  // zinv fused with calcnu
+};
+
+}  // namespace dbec

@@ -32,7 +32,7 @@ struct Buffers {
     double *rows = nullptr;
     double *red = nullptr;   // reduction scratch (>= 4 * (Nx + Ny + Nz) doubles)
     double *scal = nullptr;  // device scalars, see SC_* below
-    double *coef = nullptr;  // [axis][calpha | cgamma], real or complex, FP64
+    double *coef = nullptr;  // CN coefficients and carry multipliers (CoefLayout), real or complex, FP64
     void *coef_lo = nullptr; // same in FP32 for the mixed form
     double *wgt = nullptr;   // Simpson weights wx | wy | wz (include h/3)
     double *sq = nullptr;    // x^2 | y^2 | z^2

@@ -167,7 +167,6 @@ __device__ __forceinline__ double m_add(double a, double b) { return a + b; }
 __device__ __forceinline__ double m_sub(double a, double b) { return a - b; }
 __device__ __forceinline__ double m_fma(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ double m_zero(double) { return 0.0; }
-__device__ __forceinline__ double m_one(double) { return 1.0; }
 __device__ __forceinline__ double m_mod2(double a) { return a * a; }
 __device__ __forceinline__ cpx<double> m_mul(cpx<double> a, cpx<double> b) { return cmul(a, b); }
 __device__ __forceinline__ cpx<double> m_add(cpx<double> a, cpx<double> b) { return a + b; }
@@ -180,7 +179,6 @@ __device__ __forceinline__ cpx<double> m_fma(cpx<double> a, cpx<double> b, cpx<d
     return {re, im};
 }
 __device__ __forceinline__ cpx<double> m_zero(cpx<double>) { return {0.0, 0.0}; }
-__device__ __forceinline__ cpx<double> m_one(cpx<double>) { return {1.0, 0.0}; }
 __device__ __forceinline__ double m_mod2(cpx<double> a) { return a.x * a.x + a.y * a.y; }
 // FP32 (mixed form): the recurrences in single precision, the Simpson
 // partials of |psi|^2 in double.
@@ -189,7 +187,6 @@ __device__ __forceinline__ float m_add(float a, float b) { return a + b; }
 __device__ __forceinline__ float m_sub(float a, float b) { return a - b; }
 __device__ __forceinline__ float m_fma(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ float m_zero(float) { return 0.0f; }
-__device__ __forceinline__ float m_one(float) { return 1.0f; }
 __device__ __forceinline__ double m_mod2(float a) { return (double)a * (double)a; }
 __device__ __forceinline__ cpx<float> m_mul(cpx<float> a, cpx<float> b) { return cmul(a, b); }
 __device__ __forceinline__ cpx<float> m_add(cpx<float> a, cpx<float> b) { return a + b; }
@@ -202,7 +199,6 @@ __device__ __forceinline__ cpx<float> m_fma(cpx<float> a, cpx<float> b, cpx<floa
     return {re, im};
 }
 __device__ __forceinline__ cpx<float> m_zero(cpx<float>) { return {0.0f, 0.0f}; }
-__device__ __forceinline__ cpx<float> m_one(cpx<float>) { return {1.0f, 0.0f}; }
 __device__ __forceinline__ double m_mod2(cpx<float> a) { return (double)a.x * a.x + (double)a.y * a.y; }
 
 // Storage and arithmetic types: FP64 lines in FP64, FP32 lines (mixed form)
@@ -217,52 +213,96 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// LDS layout shared by both sweep kernels: coefficient tables (n each), the
+// Device coefficient tables of one axis: calpha and cgamma (n - 1 each), and
+// for the segment length M of the launch the carry multipliers of the
+// partitioned solve (n each):
+//   gb[i] = prod_{t = i .. top} cgamma[t] A,   fa[e] = prod_{t = bot .. e} calpha[t - 1]
+// over the in-range indices (1..n-2) of the segment holding i / e, top and
+// bot its last and first in-range index.  They are products of coefficients
+// only, formed on the host in FP64.
+template <typename A>
+struct SweepTab {
+    const A *ca, *cg, *gb, *fa;
+};
+
+// LDS layout shared by both sweep kernels: the four tables (n each), the
 // carry exchange (one per thread), the last elements, the row partials.
 template <typename A>
 struct SweepShared {
-    A *ca, *cg, *ga, *exa, *exb, *exl;
+    A *ca, *cg, *gb, *fa, *exa, *exb, *exl;
     double *exn;
     __device__ SweepShared(char *raw, int n, int nthreads, int L) {
         ca = reinterpret_cast<A *>(raw);
         cg = ca + n;
-        ga = cg + n;
-        exa = ga + n;
+        gb = cg + n;
+        fa = gb + n;
+        exa = fa + n;
         exb = exa + nthreads;
         exl = exb + nthreads;
         exn = reinterpret_cast<double *>(exl + L);
     }
     __host__ __device__ static size_t bytes(int n, int nthreads, int L) {
-        return (3 * (size_t)n + 2 * (size_t)nthreads + L) * sizeof(A) + (size_t)nthreads * sizeof(double);
+        return (4 * (size_t)n + 2 * (size_t)nthreads + L) * sizeof(A) + (size_t)nthreads * sizeof(double);
     }
 };
 
-// The recursion coefficients are staged into LDS in two parts: the first
-// slot of each thread is loaded (coef_pre) before the thread's psi loads, so
-// writing it to LDS (coef_put) waits only for that older load while the psi
-// loads are still in flight; slots beyond the first thread count follow.
+// The tables are staged into LDS in two parts: the first slot of each thread
+// is loaded (coef_pre) before the thread's psi loads, so writing it to LDS
+// (coef_put) waits only for that older load while the psi loads are still
+// in flight; slots beyond the first thread count follow.
 template <typename A>
 struct CoefPre {
-    A ca, cg, ga;
+    A ca, cg, gb, fa;
 };
 template <typename A>
-__device__ __forceinline__ CoefPre<A> coef_pre(const A *ca, const A *cg, const A *ga, int n, int tid) {
-    const int i = tid < n - 1 ? tid : 0;
-    return {ca[i], cg[i], ga[i]};
+__device__ __forceinline__ CoefPre<A> coef_pre(const SweepTab<A> &t, int n, int tid) {
+    const int i = tid < n ? tid : 0, ic = i < n - 1 ? i : 0;
+    return {t.ca[ic], t.cg[ic], t.gb[i], t.fa[i]};
 }
 template <typename A>
-__device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const A *ca, const A *cg,
-                                         const A *ga, int n, int tid, int nthreads) {
-    if (tid < n - 1) {
-        sh.ca[tid] = c.ca;
-        sh.cg[tid] = c.cg;
-        sh.ga[tid] = c.ga;
+__device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const SweepTab<A> &t, int n,
+                                         int tid, int nthreads) {
+    if (tid < n) {
+        if (tid < n - 1) {
+            sh.ca[tid] = c.ca;
+            sh.cg[tid] = c.cg;
+        }
+        sh.gb[tid] = c.gb;
+        sh.fa[tid] = c.fa;
     }
-    for (int i = tid + nthreads; i < n - 1; i += nthreads) {
-        sh.ca[i] = ca[i];
-        sh.cg[i] = cg[i];
-        sh.ga[i] = ga[i];
+    for (int i = tid + nthreads; i < n; i += nthreads) {
+        if (i < n - 1) {
+            sh.ca[i] = t.ca[i];
+            sh.cg[i] = t.cg[i];
+        }
+        sh.gb[i] = t.gb[i];
+        sh.fa[i] = t.fa[i];
     }
+}
+
+// The backward recurrence's right-hand side c_i = mA psi[i+1] + A0r psi[i] -
+// A psi[i-1] and update cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i).  Complex
+// lines (real time) have A = i a, mA = -A and A0r = 1 + i b (gencoef,
+// real3d.cu:991-1035; launch_sweep checks it), so there
+//   c = -i a (psi[i+1] + psi[i-1]) + (1 + i b) psi[i],  A cbeta - c = i a cbeta - c.
+template <typename T>
+__device__ __forceinline__ T sweep_rhs(const SweepConst<T> &k, T pn, T pc, T pp) {
+    return m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+}
+template <typename T>
+__device__ __forceinline__ T sweep_back(const SweepConst<T> &k, T cg, T bq, T c) {
+    return m_mul(cg, m_sub(m_mul(k.A, bq), c));
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> sweep_rhs(const SweepConst<cpx<R>> &k, cpx<R> pn, cpx<R> pc, cpx<R> pp) {
+    const R a = k.A.y, b = k.A0r.y;
+    const R sx = pn.x + pp.x, sy = pn.y + pp.y;
+    return {fma(a, sy, fma(-b, pc.y, pc.x)), fma(-a, sx, fma(b, pc.x, pc.y))};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> sweep_back(const SweepConst<cpx<R>> &k, cpx<R> cg, cpx<R> bq, cpx<R> c) {
+    const R a = k.A.y;
+    return cmul(cg, cpx<R>{fma(-a, bq.y, -c.x), fma(a, bq.x, -c.y)});
 }
 
 // The partitioned solve of one segment (registers v, halos hl / hr).  Every
@@ -281,15 +321,13 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     static_assert(M >= 2, "segment length");
     const bool in0 = s > 0, inM = s < P - 1;  // j = 0 / j = M-1 inside 1..n-2
     auto inr = [&](int j) { return PEEL || (e0 + j >= 1 && e0 + j <= n - 2); };  // middle iterations
-    const A zero = m_zero(k.A), one = m_one(k.A);
-    A bq = zero, bp = one;
+    const int bot = e0 > 1 ? e0 : 1, top = e0 + M - 1 < n - 2 ? e0 + M - 1 : n - 2;
+    const A zero = m_zero(k.A);
+    A bq = zero;
     {
         A pn = hr, pc = v[M - 1];
         auto back = [&](int j, A pp) {
-            const int i = e0 + j;
-            const A c = m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
-            bq = m_mul(sh.cg[i], m_sub(m_mul(k.A, bq), c));
-            bp = m_mul(sh.ga[i], bp);
+            bq = sweep_back(k, sh.cg[e0 + j], bq, sweep_rhs(k, pn, pc, pp));
             v[j] = bq;
         };
         if (inM) back(M - 1, v[M - 2]);
@@ -305,17 +343,13 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         if (in0) back(0, hl);
     }
     sh.exa[s * L + l] = bq;
-    sh.exb[s * L + l] = bp;
+    sh.exb[s * L + l] = sh.gb[bot];  // the segment's product of cgamma A
     if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
     __syncthreads();
     A x = sh.exl[l];
     for (int t = P - 1; t > s; --t) x = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], x));
     {
-        A pr = one;
-        auto fixb = [&](int j) {
-            pr = m_mul(sh.ga[e0 + j], pr);
-            v[j] = m_add(v[j], m_mul(pr, x));
-        };
+        auto fixb = [&](int j) { v[j] = m_fma(sh.gb[e0 + j], x, v[j]); };
         if (inM) fixb(M - 1);
 #pragma unroll
         for (int j = M - 2; j >= 1; --j)
@@ -323,12 +357,10 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         if (in0) fixb(0);
     }
     __syncthreads();  // carries read; the exchange is reused below
-    A fy = zero, fr = one;
+    A fy = zero;
     {
         auto fwd = [&](int j) {
-            const A a = sh.ca[e0 + j - 1];
-            fy = m_fma(a, fy, v[j]);
-            fr = m_mul(a, fr);
+            fy = m_fma(sh.ca[e0 + j - 1], fy, v[j]);
             v[j] = fy;
         };
         if (in0) fwd(0);
@@ -338,16 +370,14 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         if (inM) fwd(M - 1);
     }
     sh.exa[s * L + l] = fy;
-    sh.exb[s * L + l] = fr;
+    sh.exb[s * L + l] = sh.fa[top];  // the segment's product of calpha
     __syncthreads();
     A y = zero;  // psi'[0] = 0
     for (int t = 0; t < s; ++t) y = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], y));
     double acc = 0.0;
     {
-        A pr = one;
         auto fixf = [&](int j) {
-            pr = m_mul(sh.ca[e0 + j - 1], pr);
-            v[j] = m_add(v[j], m_mul(pr, y));
+            v[j] = m_fma(sh.fa[e0 + j], y, v[j]);
             if (COLS) acc += wz[e0 + j] * m_mod2(v[j]);
         };
         if (in0) fixf(0);
@@ -366,9 +396,8 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
 // (l, s) loads its segment straight from HBM into registers.
 template <typename S, int AXIS, int L, int M>
 __global__ void __launch_bounds__(512)
-k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
-           const typename Arith<S>::T *__restrict__ calpha, const typename Arith<S>::T *__restrict__ cgamma,
-           const typename Arith<S>::T *__restrict__ ga, SweepConst<typename Arith<S>::T> k) {
+k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
+           SweepConst<typename Arith<S>::T> k) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
     S *__restrict__ psi = reinterpret_cast<S *>(solve_at(sv).psi);
@@ -386,7 +415,7 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
     const long es = AXIS == 0 ? (long)Ny * Nz : (long)Nz;
     const int e0 = s * M;
     const A zero = m_zero(k.A);
-    const CoefPre<A> cp = coef_pre(calpha, cgamma, ga, n, tid);
+    const CoefPre<A> cp = coef_pre(tab, n, tid);
     // Predicated loads converted in place: fewer registers (higher occupancy)
     // than issuing them into a separate array, which measured faster here.
     A v[M];
@@ -394,7 +423,7 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
     for (int j = 0; j < M; ++j) v[j] = act ? to_a(psi[base + (long)(e0 + j) * es]) : zero;
     const A hl = (act && e0 > 0) ? to_a(psi[base + (long)(e0 - 1) * es]) : zero;
     const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
-    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
+    coef_put(sh, cp, tab, n, tid, nt);
     __syncthreads();
     seg_solve<A, M, false, std::is_floating_point<A>::value>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
@@ -412,10 +441,8 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz,
 // colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
 template <typename S, int L, int M, bool COLS>
 __global__ void __launch_bounds__(512)
-k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
-          const typename Arith<S>::T *__restrict__ calpha, const typename Arith<S>::T *__restrict__ cgamma,
-          const typename Arith<S>::T *__restrict__ ga, SweepConst<typename Arith<S>::T> k,
-          const double *__restrict__ wz, const double *__restrict__ wy) {
+k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
+          SweepConst<typename Arith<S>::T> k, const double *__restrict__ wz, const double *__restrict__ wy) {
     using A = typename Arith<S>::T;
     extern __shared__ char smem_raw[];
     const SolveAt sr = solve_at(sv);
@@ -439,14 +466,14 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz,
     constexpr int VE = Pk<S>::n, KV = M / VE;
     static_assert(M % VE == 0, "z sweep packets");
     const int npk = nl * n / VE;
-    const CoefPre<A> cp = coef_pre(calpha, cgamma, ga, n, tid);
+    const CoefPre<A> cp = coef_pre(tab, n, tid);
     typename Pk<S>::type r[KV];
 #pragma unroll
     for (int q = 0; q < KV; ++q) {
         const int iv = tid + q * nt;
         r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
     }
-    coef_put(sh, cp, calpha, cgamma, ga, n, tid, nt);
+    coef_put(sh, cp, tab, n, tid, nt);
 #pragma unroll
     for (int q = 0; q < KV; ++q) {
         const int iv = tid + q * nt;
@@ -546,9 +573,18 @@ struct Launch {
     Solves v;
 };
 
+// The tables of one axis for both segment lengths (16 and 32).
+template <typename A>
+struct SweepTabs {
+    const A *ca, *cg, *gb[2], *fa[2];
+    SweepTab<A> at(int M) const {
+        const int m = M == 32 ? 1 : 0;
+        return {ca, cg, gb[m], fa[m]};
+    }
+};
+
 template <typename S, int AXIS, int L>
-static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                  const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
                                   const SweepConst<typename Arith<S>::T> &k, int M) {
     using A = typename Arith<S>::T;
     const int n = AXIS == 0 ? Nx : Ny;
@@ -560,13 +596,13 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
     const size_t lds = SweepShared<A>::bytes(n, L * P, L);
     if constexpr (sizeof(A) <= 8) {
         if (M == 32) {
-            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
+            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(32), k);
             return hipGetLastError();
         }
     } else {
         if (M != 16) return hipErrorInvalidValue;
     }
-    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k);
+    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(16), k);
     return hipGetLastError();
 }
 
@@ -574,8 +610,7 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
 // elements; FP32 real lines use 128-byte rows and 16-element segments.
 // DBEC_SWEEP_XY_{ROWB,M}[32] override (tuning runs).
 template <typename S, int AXIS>
-static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                    const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
                                     const SweepConst<typename Arith<S>::T> &k) {
     using A = typename Arith<S>::T;
     // FP32 real lines: 128-byte rows and 16-element segments (measured at 256^3)
@@ -584,13 +619,12 @@ static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, con
     const int n = AXIS == 0 ? Nx : Ny;
     int M = (mset == 16 || mset == 32) ? mset : seg_len<A>(n);
     if (sizeof(A) > 8) M = 16;
-    if (rowb == 128) return sweep_xy_launch<S, AXIS, 128 / sizeof(S)>(q, Nx, Ny, Nz, ca, cg, ga, k, M);
-    return sweep_xy_launch<S, AXIS, 256 / sizeof(S)>(q, Nx, Ny, Nz, ca, cg, ga, k, M);
+    if (rowb == 128) return sweep_xy_launch<S, AXIS, 128 / sizeof(S)>(q, Nx, Ny, Nz, t, k, M);
+    return sweep_xy_launch<S, AXIS, 256 / sizeof(S)>(q, Nx, Ny, Nz, t, k, M);
 }
 
 template <typename S, int L, bool COLS>
-static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                 const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const SweepTab<typename Arith<S>::T> &t,
                                  const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
     using A = typename Arith<S>::T;
     const int P = Nz / 16;
@@ -602,75 +636,72 @@ static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const 
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t, k, wz, wy);
     return hipGetLastError();
 }
 
 template <typename S, bool COLS>
-static hipError_t sweep_z_dispatch(const Launch &q, int Nx, int Ny, int Nz, const typename Arith<S>::T *ca,
-                                   const typename Arith<S>::T *cg, const typename Arith<S>::T *ga,
+static hipError_t sweep_z_dispatch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &ts,
                                    const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
     if (Nz % 16 != 0) return hipErrorInvalidValue;
+    const SweepTab<typename Arith<S>::T> t = ts.at(16);
     switch (z_rows<S>(Nz)) {
-    case 64: return sweep_z_launch<S, 64, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
-    case 32: return sweep_z_launch<S, 32, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
-    case 16: return sweep_z_launch<S, 16, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
-    case 8: return sweep_z_launch<S, 8, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
-    default: return sweep_z_launch<S, 4, COLS>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    case 64: return sweep_z_launch<S, 64, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 32: return sweep_z_launch<S, 32, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 16: return sweep_z_launch<S, 16, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 8: return sweep_z_launch<S, 8, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
+    default: return sweep_z_launch<S, 4, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
     }
 }
 
 template <typename S, typename A>
-static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const A *ca, const A *cg, const A *ga,
-                             const SweepConst<A> &k, bool cols) {
+static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const A *base, const SweepConst<A> &k,
+                             bool cols) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
+    const CoefLayout lay(c->Nx, c->Ny, c->Nz);
+    const SweepTabs<A> t{base + lay.ca(axis), base + lay.cg(axis),
+                         {base + lay.gb(axis, 0), base + lay.gb(axis, 1)},
+                         {base + lay.fa(axis, 0), base + lay.fa(axis, 1)}};
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
-    if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, ca, cg, ga, k);
-    if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, ca, cg, ga, k);
-    if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
-    return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, ca, cg, ga, k, wz, wy);
+    if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, t, k);
+    if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, t, k);
+    if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, t, k, wz, wy);
+    return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, t, k, wz, wy);
 }
 
 hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves &v, bool cols) {
     const auto &b = c->b;
     const Launch q{s, v};
-    const long off[3] = {0, c->Nx - 1, c->Nx - 1 + c->Ny - 1};
-    const long tot = (c->Nx - 1) + (c->Ny - 1) + (c->Nz - 1);
     const bool f32 = c->prec == DBEC_MIXED;
     if (c->mode == DBEC_IMAG) {
         if (f32) {
-            const float *cf = static_cast<const float *>(b.coef_lo);
             SweepConst<float> k;
             k.A0r = (float)c->A0r[axis][0];
             k.A = (float)c->A[axis][0];
             k.mA = -k.A;
-            return sweep_axis<float>(q, c, axis, cf + off[axis], cf + tot + off[axis], cf + 2 * tot + off[axis], k,
-                                     cols);
+            return sweep_axis<float>(q, c, axis, static_cast<const float *>(b.coef_lo), k, cols);
         }
-        const double *cd = b.coef;
         SweepConst<double> k;
         k.A0r = c->A0r[axis][0];
         k.A = c->A[axis][0];
         k.mA = -c->A[axis][0];
-        return sweep_axis<double>(q, c, axis, cd + off[axis], cd + tot + off[axis], cd + 2 * tot + off[axis], k,
-                                  cols);
+        return sweep_axis<double>(q, c, axis, b.coef, k, cols);
     }
+    // complex lines: the sweep kernels use A = i a, mA = -A, A0r = 1 + i b
+    if (c->A[axis][0] != 0.0 || c->A0r[axis][0] != 1.0 || c->mA[axis][0] != 0.0 || c->mA[axis][1] != -c->A[axis][1])
+        return hipErrorInvalidValue;
     if (f32) {
-        const cpx<float> *cf = static_cast<const cpx<float> *>(b.coef_lo);
         SweepConst<cpx<float>> k;
         k.A0r = {(float)c->A0r[axis][0], (float)c->A0r[axis][1]};
         k.A = {(float)c->A[axis][0], (float)c->A[axis][1]};
         k.mA = {(float)c->mA[axis][0], (float)c->mA[axis][1]};
-        return sweep_axis<cpx<float>>(q, c, axis, cf + off[axis], cf + tot + off[axis], cf + 2 * tot + off[axis], k,
-                                      cols);
+        return sweep_axis<cpx<float>>(q, c, axis, static_cast<const cpx<float> *>(b.coef_lo), k, cols);
     }
-    const cpx<double> *cc = reinterpret_cast<const cpx<double> *>(b.coef);
     SweepConst<cpx<double>> k;
     k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};
     k.A = {c->A[axis][0], c->A[axis][1]};
     k.mA = {c->mA[axis][0], c->mA[axis][1]};
-    return sweep_axis<cpx<double>>(q, c, axis, cc + off[axis], cc + tot + off[axis], cc + 2 * tot + off[axis], k,
-                                   cols);
+    return sweep_axis<cpx<double>>(q, c, axis, reinterpret_cast<const cpx<double> *>(b.coef), k, cols);
 }
 
 // norm^2 = sum_x wx[x] sum_t colpart[t][x]; scal[NORM] = norm, scal[SCALE] = 1/norm.

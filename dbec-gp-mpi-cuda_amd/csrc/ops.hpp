@@ -17,6 +17,22 @@ struct SweepConst {
     V A0r, A, mA;  // diagonal of the explicit half, off-diagonal, and its negative
 };
 
+// Layout of the Crank-Nicolson coefficient buffers (b.coef in FP64, b.coef_lo
+// in FP32, elements of the line's value type): calpha and cgamma of the axes
+// x, y, z back to back (n - 1 each), then the carry multipliers gb and fa of
+// the partitioned sweeps (n each per axis) for segments of 16 and of 32:
+//   [calpha | cgamma | gb16 | fa16 | gb32 | fa32]
+struct CoefLayout {
+    long tot, totn, off[3], offn[3];
+    CoefLayout(long Nx, long Ny, long Nz)
+        : tot(Nx + Ny + Nz - 3), totn(Nx + Ny + Nz), off{0, Nx - 1, Nx - 1 + Ny - 1}, offn{0, Nx, Nx + Ny} {}
+    long ca(int a) const { return off[a]; }
+    long cg(int a) const { return tot + off[a]; }
+    long gb(int a, int m) const { return 2 * tot + 2 * m * totn + offn[a]; }  // m = 0: M = 16, 1: M = 32
+    long fa(int a, int m) const { return 2 * tot + (2 * m + 1) * totn + offn[a]; }
+    long size() const { return 2 * tot + 4 * totn; }
+};
+
 hipError_t launch_facefix(hipStream_t s, void *phi, bool fp32, long Nx, long Ny, long Nz);
 hipError_t launch_calcnu(hipStream_t s, const dbec_ctx *c, const double *scale);
 hipError_t launch_scale(hipStream_t s, const dbec_ctx *c, const double *scale);
