@@ -4,9 +4,13 @@
 // real3d.cu = src/real3d-mpicuda/real3d-mpicuda.cu.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "core.hpp"
@@ -158,6 +162,7 @@ FftArgs fft_args(const dbec_ctx *c, int axis, const Solves &v) {
     a.nc = nu_const(c);
     a.wx = c->b.wgt;
     a.tiles = (int)sweep_z_tiles(c);
+    a.tune = c->tune;
     return a;
 }
 
@@ -478,6 +483,99 @@ int setup(dbec_ctx *c) {
     return DBEC_OK;
 }
 
+// Autotuning of the launch choices in Tuning: each candidate runs a few
+// times on the context's stream (psi and the spectrum are zero: the kernels
+// move the same bytes and do the same arithmetic as on a state) and the
+// fastest is kept, pass by pass.  The choices never change a result, only
+// its speed.  Results are kept per (grid, mode, precision, device) for the
+// process, so contexts of one shape share them.  DBEC_AUTOTUNE=0 keeps the
+// built-in rules; DBEC_AUTOTUNE_LOG=1 prints the choices to stderr.
+struct TuneKey {
+    long nx, ny, nz;
+    int mode, prec, dev;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(nx, ny, nz, mode, prec, dev) < std::tie(o.nx, o.ny, o.nz, o.mode, o.prec, o.dev);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tuning> g_tune;
+
+// Best of `trials` timings of `reps` back-to-back calls of f, in ms.
+template <typename F>
+int time_launches(dbec_ctx *c, F &&f, double *ms) {
+    constexpr int trials = 2, reps = 3;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0), "event");
+    HIPCHK(c, hipEventCreate(&e1), "event");
+    double best = 1e30;
+    hipError_t err = f();  // warm-up (first launch of the variant)
+    for (int t = 0; t < trials && err == hipSuccess; t++) {
+        err = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && err == hipSuccess; r++) err = f();
+        if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float v = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&v, e0, e1);
+        if (err == hipSuccess) best = std::min(best, (double)v / reps);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(c, err, "autotune launch");
+    *ms = best;
+    return DBEC_OK;
+}
+
+int autotune(dbec_ctx *c) {
+    const char *env = getenv("DBEC_AUTOTUNE");
+    if (env && std::strcmp(env, "0") == 0) return DBEC_OK;
+    const TuneKey key{c->Nx, c->Ny, c->Nz, (int)c->mode, (int)c->prec, c->device};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            c->tune = it->second;
+            return DBEC_OK;
+        }
+    }
+    const Solves v = solve_ref(c, false);
+    // pick the fastest value of one field among the candidates
+    auto choose = [&](int *field, std::initializer_list<int> cands, auto &&launch) -> int {
+        double best = 1e30;
+        int pick = *field;
+        for (int cand : cands) {
+            *field = cand;
+            double ms;
+            CHK(time_launches(c, launch, &ms));
+            if (ms < best) {
+                best = ms;
+                pick = cand;
+            }
+        }
+        *field = pick;
+        return DBEC_OK;
+    };
+    Tuning &t = c->tune;
+    CHK(choose(&t.lm[0], {1, 2}, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }));
+    CHK(choose(&t.lm[1], {1, 2}, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }));
+    CHK(choose(&t.lm[2], {1, 2}, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }));
+    CHK(choose(&t.lm[3], {1, 2}, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }));
+    for (int axis = 0; axis < 2; axis++) {
+        auto sweep = [&] { return launch_sweep(c->stream, c, axis, v, false); };
+        CHK(choose(&t.xy_peel[axis], {0, 1}, sweep));
+        CHK(choose(&t.xy_rowb[axis], {128, 256}, sweep));
+    }
+    // the candidates ran on zero data; psi and the spectrum stay zero
+    if (const char *lg = getenv("DBEC_AUTOTUNE_LOG"); lg && std::strcmp(lg, "0") != 0)
+        std::fprintf(stderr, "[dbec autotune] %ldx%ldx%ld %s %s: lm zfwd %d ypass %d xconv %d fused %d; "
+                             "sweep x rowb %d peel %d, y rowb %d peel %d\n",
+                     c->Nx, c->Ny, c->Nz, c->mode == DBEC_IMAG ? "imag" : "real",
+                     c->prec == DBEC_MIXED ? "mixed" : "fp64", t.lm[0], t.lm[1], t.lm[2], t.lm[3], t.xy_rowb[0],
+                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1]);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = t;
+    return DBEC_OK;
+}
+
 // Host-side selection of every o-th entry of a row-major 2-D array.
 void pick2(const std::vector<double> &m, long n0, long n1, long o0, long o1, double *out) {
     long w = 0;
@@ -539,6 +637,7 @@ int dbec_create(const dbec_params *p, dbec_mode mode, dbec_prec prec, void *stre
         c->own_stream = true;
     }
     int rc = setup(c);
+    if (rc == DBEC_OK) rc = autotune(c);
     if (rc != DBEC_OK) {
         std::string m = c->err;
         (void)hipStreamSynchronize(c->stream);

@@ -504,9 +504,15 @@ struct FftLaunchImpl {
         }
         return hipGetLastError();
     }
+    // The tile multiplier of a pass: the environment variable, else the
+    // context's tuned choice, else the rule.
+    static int pick_lm(const FftArgs &a, int env, int pass, int rule) {
+        if (env) return env;
+        return a.tune.lm[pass] ? a.tune.lm[pass] : rule;
+    }
     static hipError_t zfwd(const FftArgs &a) {
-        static const int l64 = fft_lm("DBEC_FFT_LM_ZFWD64", 1), l32 = fft_lm("DBEC_FFT_LM_ZFWD32", WIDE);
-        return (a.fp32 ? l32 : l64) == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
+        static const int l64 = fft_lm("DBEC_FFT_LM_ZFWD64", 0), l32 = fft_lm("DBEC_FFT_LM_ZFWD32", 0);
+        return pick_lm(a, a.fp32 ? l32 : l64, 0, a.fp32 ? WIDE : 1) == 2 ? zfwd_t<2>(a) : zfwd_t<1>(a);
     }
     // FP32 spectra use tiles twice as wide, so every strided access still
     // moves 128-256 contiguous bytes.
@@ -528,8 +534,8 @@ struct FftLaunchImpl {
         return hipGetLastError();
     }
     static hipError_t ypass(int dir, const FftArgs &a) {
-        static const int l64 = fft_lm("DBEC_FFT_LM_YPASS64", 1), l32 = fft_lm("DBEC_FFT_LM_YPASS32", WIDE);
-        return (a.fp32 ? l32 : l64) == 2 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a);
+        static const int l64 = fft_lm("DBEC_FFT_LM_YPASS64", 0), l32 = fft_lm("DBEC_FFT_LM_YPASS32", 0);
+        return pick_lm(a, a.fp32 ? l32 : l64, 1, a.fp32 ? WIDE : 1) == 2 ? ypass_t<2>(dir, a) : ypass_t<1>(dir, a);
     }
 
     template <int LM>
@@ -546,8 +552,8 @@ struct FftLaunchImpl {
         return hipGetLastError();
     }
     static hipError_t xconv(const FftArgs &a) {
-        static const int l64 = fft_lm("DBEC_FFT_LM_XCONV64", 1), l32 = fft_lm("DBEC_FFT_LM_XCONV32", 2);
-        return (a.fp32 ? l32 : l64) == 2 ? xconv_t<2>(a) : xconv_t<1>(a);
+        static const int l64 = fft_lm("DBEC_FFT_LM_XCONV64", 0), l32 = fft_lm("DBEC_FFT_LM_XCONV32", 0);
+        return pick_lm(a, a.fp32 ? l32 : l64, 2, a.fp32 ? 2 : 1) == 2 ? xconv_t<2>(a) : xconv_t<1>(a);
     }
 
     // single solve: spectrum of `one` -> phi
@@ -584,8 +590,8 @@ struct FftLaunchImpl {
     }
     static hipError_t fused(const FftArgs &a) {
         // FP64: narrow tiles for complex psi
-        static const int l64 = fft_lm("DBEC_FFT_LM_FUSED64", 0), l32 = fft_lm("DBEC_FFT_LM_FUSED32", WIDE);
-        const int lm = a.fp32 ? l32 : (l64 ? l64 : (a.cpsi ? 1 : WIDE));
+        static const int l64 = fft_lm("DBEC_FFT_LM_FUSED64", 0), l32 = fft_lm("DBEC_FFT_LM_FUSED32", 0);
+        const int lm = pick_lm(a, a.fp32 ? l32 : l64, 3, (a.fp32 || !a.cpsi) ? WIDE : 1);
         return lm == 2 ? fused_lm<2>(a) : fused_lm<1>(a);
     }
 };

@@ -47,6 +47,15 @@ struct Buffers {
 enum { SC_NORM = 0, SC_SCALE = 1, SC_ONE = 2, SC_MU = 3, SC_EN = 4, SC_R0 = 5, SC_RX = 6, SC_RY = 7, SC_RZ = 8,
        SC_SHIFT = 9, SC_LASTNORM = 10, SC_CAPPED = 11, SC_COUNT = 16 };
 
+// Launch choices of the step kernels that leave every result bit-identical
+// (which lines share a workgroup, which iterations test their range), picked
+// per context by timing (autotune at creation); 0 / -1: the built-in rule.
+struct Tuning {
+    int lm[4] = {0, 0, 0, 0};   // FFT tile multiplier (1, 2): zfwd, ypass, xconv, fused inverse z
+    int xy_rowb[2] = {0, 0};    // x / y sweeps: bytes of z per row load (128, 256)
+    int xy_peel[2] = {-1, -1};  // x / y sweeps: boundary iterations peeled (1) or tested (0)
+};
+
 struct KernelTimer;
 
 }  // namespace dbec
@@ -69,6 +78,7 @@ struct dbec_ctx {
     bool view = false;           // buffers belong to another owner (request-loop slot)
     bool phi_valid = false;
     dbec::Buffers b;
+    dbec::Tuning tune;
     dbec::KernelTimer *timer = nullptr;  // set while dbec_profile_step records
     std::string err;
 };
@@ -136,6 +146,7 @@ struct FftArgs {
     NuConst nc;           // dt and grid size; couplings come from the solve
     const double *wx;     // Simpson weights along x (folded norm)
     int tiles;            // rows of the column partials (folded norm)
+    Tuning tune;          // the context's launch choices
 };
 // Each returns hipSuccess or an error; `n` is the transformed length.
 hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C along z)

@@ -394,7 +394,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
 // x and y sweeps: a workgroup covers L consecutive z (lanes; one 256-byte row
 // per load) of the lines of one y (x sweep) or x (y sweep) plane; thread
 // (l, s) loads its segment straight from HBM into registers.
-template <typename S, int AXIS, int L, int M>
+template <typename S, int AXIS, int L, int M, bool PEEL>
 __global__ void __launch_bounds__(512)
 k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
            SweepConst<typename Arith<S>::T> k) {
@@ -425,7 +425,7 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
     const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
     coef_put(sh, cp, tab, n, tid, nt);
     __syncthreads();
-    seg_solve<A, M, false, std::is_floating_point<A>::value>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
+    seg_solve<A, M, false, PEEL>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
 #pragma unroll
         for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
@@ -528,11 +528,17 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     }
 }
 
-// Segment length of a line of n elements: 32 for complex FP32 lines that
-// split into at least 4 segments, else 16 (measured at 256^3 and 192^3).
-template <typename A>
+// Segment length of the x / y sweeps for a line of n values of S: 32 for
+// complex FP32 lines that split into at least 6 segments of 32 (measured at
+// 96^3 to 256^3) and for lines whose 16-element segments would need more
+// than 512 threads even with 128-byte rows; else 16.  It fixes the rounding
+// of the partitioned solve, so it depends on the line only (never on a tuned
+// choice).
+template <typename S>
 static int seg_len(long n) {
-    if (!std::is_floating_point<A>::value && sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 4) return 32;
+    using A = typename Arith<S>::T;
+    if (!std::is_floating_point<A>::value && sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 6) return 32;
+    if ((n / 16) * (long)(128 / sizeof(S)) > 512) return 32;
     return 16;
 }
 
@@ -583,12 +589,11 @@ struct SweepTabs {
     }
 };
 
-template <typename S, int AXIS, int L>
+template <typename S, int AXIS, int L, bool PEEL>
 static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
                                   const SweepConst<typename Arith<S>::T> &k, int M) {
     using A = typename Arith<S>::T;
     const int n = AXIS == 0 ? Nx : Ny;
-    if (L * (n / M) > 512) M = 32;
     const int P = n / M;
     if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
     const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
@@ -596,31 +601,43 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
     const size_t lds = SweepShared<A>::bytes(n, L * P, L);
     if constexpr (sizeof(A) <= 8) {
         if (M == 32) {
-            k_sweep_xy<S, AXIS, L, 32><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(32), k);
+            k_sweep_xy<S, AXIS, L, 32, PEEL><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(32), k);
             return hipGetLastError();
         }
     } else {
         if (M != 16) return hipErrorInvalidValue;
     }
-    k_sweep_xy<S, AXIS, L, 16><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(16), k);
+    k_sweep_xy<S, AXIS, L, 16, PEEL><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(16), k);
     return hipGetLastError();
 }
 
-// x / y sweeps: rows of 256 bytes of z per load, segments of seg_len
-// elements; FP32 real lines use 128-byte rows and 16-element segments.
-// DBEC_SWEEP_XY_{ROWB,M}[32] override (tuning runs).
+// x / y sweeps: segments of seg_len elements; per axis the row width (bytes
+// of z per load: 256, or 128 for FP32 real lines) and the boundary peeling
+// (real lines) by rule, or as tuned for the context.  DBEC_SWEEP_XY_{ROWB,M}
+// (FP64 and complex lines) and DBEC_SWEEP_XY_{ROWB,M}32 (FP32 real lines)
+// override (tuning runs).
+template <typename S, int AXIS, bool PEEL>
+static hipError_t sweep_xy_rows(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
+                                const SweepConst<typename Arith<S>::T> &k, int rowb, int M) {
+    const int n = AXIS == 0 ? Nx : Ny;
+    // 256-byte rows only where their workgroup stays within 512 threads
+    if (rowb == 128 || (long)(256 / sizeof(S)) * (n / M) > 512)
+        return sweep_xy_launch<S, AXIS, 128 / sizeof(S), PEEL>(q, Nx, Ny, Nz, t, k, M);
+    return sweep_xy_launch<S, AXIS, 256 / sizeof(S), PEEL>(q, Nx, Ny, Nz, t, k, M);
+}
 template <typename S, int AXIS>
 static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
-                                    const SweepConst<typename Arith<S>::T> &k) {
+                                    const SweepConst<typename Arith<S>::T> &k, const Tuning &tu) {
     using A = typename Arith<S>::T;
-    // FP32 real lines: 128-byte rows and 16-element segments (measured at 256^3)
-    static const int rowb = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_ROWB32", 128) : env_int("DBEC_SWEEP_XY_ROWB", 256);
-    static const int mset = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_M32", 16) : env_int("DBEC_SWEEP_XY_M", 0);
+    static const int rowb_env = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_ROWB32", 0) : env_int("DBEC_SWEEP_XY_ROWB", 0);
+    static const int mset = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_M32", 0) : env_int("DBEC_SWEEP_XY_M", 0);
     const int n = AXIS == 0 ? Nx : Ny;
-    int M = (mset == 16 || mset == 32) ? mset : seg_len<A>(n);
+    int M = (mset == 16 || mset == 32) ? mset : seg_len<S>(n);
     if (sizeof(A) > 8) M = 16;
-    if (rowb == 128) return sweep_xy_launch<S, AXIS, 128 / sizeof(S)>(q, Nx, Ny, Nz, t, k, M);
-    return sweep_xy_launch<S, AXIS, 256 / sizeof(S)>(q, Nx, Ny, Nz, t, k, M);
+    const int rowb = rowb_env ? rowb_env : (tu.xy_rowb[AXIS] ? tu.xy_rowb[AXIS] : (sizeof(S) == 4 ? 128 : 256));
+    const bool peel = tu.xy_peel[AXIS] >= 0 ? tu.xy_peel[AXIS] != 0 : std::is_floating_point<A>::value;
+    if (peel) return sweep_xy_rows<S, AXIS, true>(q, Nx, Ny, Nz, t, k, rowb, M);
+    return sweep_xy_rows<S, AXIS, false>(q, Nx, Ny, Nz, t, k, rowb, M);
 }
 
 template <typename S, int L, bool COLS>
@@ -663,8 +680,8 @@ static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const
                          {base + lay.gb(axis, 0), base + lay.gb(axis, 1)},
                          {base + lay.fa(axis, 0), base + lay.fa(axis, 1)}};
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
-    if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, t, k);
-    if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, t, k);
+    if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, t, k, c->tune);
+    if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, t, k, c->tune);
     if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, t, k, wz, wy);
     return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, t, k, wz, wy);
 }

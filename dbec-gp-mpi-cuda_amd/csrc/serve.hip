@@ -246,6 +246,7 @@ struct dbec_server {
             std::memcpy(v->A0r, t->A0r, sizeof v->A0r);
             std::memcpy(v->A, t->A, sizeof v->A);
             std::memcpy(v->mA, t->mA, sizeof v->mA);
+            v->tune = t->tune;
             v->b = t->b;  // tables, phi and reduction scratch of the bucket
             v->b.psi = b->psi + (size_t)s * b->psi_st;
             v->b.spec = b->spec + (size_t)s * b->spec_st;
