@@ -614,8 +614,8 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
 // x / y sweeps: segments of seg_len elements; per axis the row width (bytes
 // of z per load: 256, or 128 for FP32 real lines) and the boundary peeling
 // (real lines) by rule, or as tuned for the context.  DBEC_SWEEP_XY_{ROWB,M}
-// (FP64 and complex lines) and DBEC_SWEEP_XY_{ROWB,M}32 (FP32 real lines)
-// override (tuning runs).
+// (FP64 and complex lines), DBEC_SWEEP_XY_{ROWB,M}32 (FP32 real lines) and
+// DBEC_SWEEP_XY_PEEL (0 or 1) override (tuning runs).
 template <typename S, int AXIS, bool PEEL>
 static hipError_t sweep_xy_rows(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &t,
                                 const SweepConst<typename Arith<S>::T> &k, int rowb, int M) {
@@ -631,11 +631,13 @@ static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, con
     using A = typename Arith<S>::T;
     static const int rowb_env = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_ROWB32", 0) : env_int("DBEC_SWEEP_XY_ROWB", 0);
     static const int mset = sizeof(S) == 4 ? env_int("DBEC_SWEEP_XY_M32", 0) : env_int("DBEC_SWEEP_XY_M", 0);
+    static const int peel_env = env_int("DBEC_SWEEP_XY_PEEL", -1);
     const int n = AXIS == 0 ? Nx : Ny;
     int M = (mset == 16 || mset == 32) ? mset : seg_len<S>(n);
     if (sizeof(A) > 8) M = 16;
     const int rowb = rowb_env ? rowb_env : (tu.xy_rowb[AXIS] ? tu.xy_rowb[AXIS] : (sizeof(S) == 4 ? 128 : 256));
-    const bool peel = tu.xy_peel[AXIS] >= 0 ? tu.xy_peel[AXIS] != 0 : std::is_floating_point<A>::value;
+    const int peel_t = peel_env >= 0 ? peel_env : tu.xy_peel[AXIS];
+    const bool peel = peel_t >= 0 ? peel_t != 0 : std::is_floating_point<A>::value;
     if (peel) return sweep_xy_rows<S, AXIS, true>(q, Nx, Ny, Nz, t, k, rowb, M);
     return sweep_xy_rows<S, AXIS, false>(q, Nx, Ny, Nz, t, k, rowb, M);
 }
