@@ -377,7 +377,9 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         const int y0 = (b % ytiles) * 2 * L;
         const int nrows = min(2 * L, Ny - y0);
         P *dst = psi + ((long)x * Ny + y0) * N;
-        if (tid < 2 * L) sqr[tid] = trap_xy(tc, sq[x], sq[Nx + min(y0 + tid, Ny - 1)]);
+        // 2L rows; a workgroup of L * T threads covers them in one pass when
+        // T >= 2, in two when T = 1 (N = 16)
+        for (int r = tid; r < 2 * L; r += WG) sqr[r] = trap_xy(tc, sq[x], sq[Nx + min(y0 + r, Ny - 1)]);
         P pv[C];
         if constexpr (PREFETCH) {
 #pragma unroll

@@ -204,6 +204,41 @@ def test_imag_512_fp64_and_mixed(hip_lib, oracle_lib):
         assert abs(mum - mud) / abs(mud) < 1e-6
 
 
+# Every supported transform length on every axis: grid i takes LENGTHS[i],
+# LENGTHS[i + 5] and LENGTHS[i + 9] (mod 13) for x, y, z, so each length's
+# radix plan runs in the z passes (zfwd, zinv, fused inverse), the y passes
+# and the x convolution, and its sweeps along each axis.
+LENGTHS = (16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512)
+ROTATED = [(LENGTHS[i], LENGTHS[(i + 5) % 13], LENGTHS[(i + 9) % 13]) for i in range(13)]
+
+
+@pytest.mark.parametrize("n", ROTATED, ids=lambda n: "x".join(map(str, n)))
+def test_every_length_on_every_axis(hip_lib, oracle_lib, n):
+    """calcpsidd2 and one composite step against the oracle (FP64, imaginary
+    and real time, 1e-12), and one mixed-precision step against FP64."""
+    say(f"every length on every axis: grid {n}")
+    p = params(n=n, h=tuple(16.0 / k for k in n), dt=0.002, cutoff=6.0)
+    for mode in (dbecpy.IMAG, dbecpy.REAL):
+        f = smooth_field(p, seed=3, complex_=(mode == dbecpy.REAL))
+        with dbecpy.Solver(hip_lib, p, mode) as g, dbecpy.Solver(oracle_lib, p, mode) as o:
+            for s in (g, o):
+                s.set_psi(f)
+                s.set_coupling(40.0, 30.0)
+                s.calcpsidd2()
+            assert rel(g.get_psidd2(), o.get_psidd2()) < TOL_OP, ("calcpsidd2", mode)
+            a, b = g.step(1), o.step(1)
+            if mode == dbecpy.IMAG:
+                assert a == pytest.approx(b, rel=TOL_OP)
+            assert rel(g.get_psi(), o.get_psi()) < TOL_OP, ("step", mode)
+            with dbecpy.Solver(hip_lib, p, mode, dbecpy.MIXED) as m:
+                m.set_psi(f)
+                m.set_coupling(40.0, 30.0)
+                m.step(1)
+                d = rel(m.get_psi(), g.get_psi())
+                say(f"  mode {mode}: mixed step vs FP64 {d:.2e}")
+                assert d < 1e-5, ("mixed step", mode)
+
+
 def test_reference_chain_product_programs(tmp_path):
     """The product's imag3d -> real3d chain on the reference's example inputs
     against the reference's shipped outputs."""
