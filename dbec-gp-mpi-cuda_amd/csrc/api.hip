@@ -564,13 +564,14 @@ int autotune(dbec_ctx *c) {
         CHK(choose(&t.xy_peel[axis], {0, 1}, sweep));
         CHK(choose(&t.xy_rowb[axis], {128, 256}, sweep));
     }
+    CHK(choose(&t.z_direct, {0, 1}, [&] { return launch_sweep(c->stream, c, 2, v, c->mode == DBEC_IMAG); }));
     // the candidates ran on zero data; psi and the spectrum stay zero
     if (const char *lg = getenv("DBEC_AUTOTUNE_LOG"); lg && std::strcmp(lg, "0") != 0)
         std::fprintf(stderr, "[dbec autotune] %ldx%ldx%ld %s %s: lm zfwd %d ypass %d xconv %d fused %d; "
-                             "sweep x rowb %d peel %d, y rowb %d peel %d\n",
+                             "sweep x rowb %d peel %d, y rowb %d peel %d, z direct %d\n",
                      c->Nx, c->Ny, c->Nz, c->mode == DBEC_IMAG ? "imag" : "real",
                      c->prec == DBEC_MIXED ? "mixed" : "fp64", t.lm[0], t.lm[1], t.lm[2], t.lm[3], t.xy_rowb[0],
-                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1]);
+                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1], t.z_direct);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = t;
     return DBEC_OK;

@@ -54,6 +54,7 @@ struct Tuning {
     int lm[4] = {0, 0, 0, 0};   // FFT tile multiplier (1, 2): zfwd, ypass, xconv, fused inverse z
     int xy_rowb[2] = {0, 0};    // x / y sweeps: bytes of z per row load (128, 256)
     int xy_peel[2] = {-1, -1};  // x / y sweeps: boundary iterations peeled (1) or tested (0)
+    int z_direct = -1;          // z sweep: segments loaded straight from HBM (1) or staged through LDS (0)
 };
 
 struct KernelTimer;

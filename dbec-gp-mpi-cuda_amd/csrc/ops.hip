@@ -285,13 +285,15 @@ __device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre
 // lines (real time) have A = i a, mA = -A and A0r = 1 + i b (gencoef,
 // real3d.cu:991-1035; launch_sweep checks it), so there
 //   c = -i a (psi[i+1] + psi[i-1]) + (1 + i b) psi[i],  A cbeta - c = i a cbeta - c.
+// Real lines (imaginary time): mA = -A exactly, so c = mA (psi[i+1] +
+// psi[i-1]) + A0r psi[i]; both forms are written as explicit FMAs.
 template <typename T>
 __device__ __forceinline__ T sweep_rhs(const SweepConst<T> &k, T pn, T pc, T pp) {
-    return m_sub(m_add(m_mul(k.mA, pn), m_mul(k.A0r, pc)), m_mul(k.A, pp));
+    return fma(k.mA, pn + pp, k.A0r * pc);
 }
 template <typename T>
 __device__ __forceinline__ T sweep_back(const SweepConst<T> &k, T cg, T bq, T c) {
-    return m_mul(cg, m_sub(m_mul(k.A, bq), c));
+    return cg * fma(k.A, bq, -c);
 }
 template <typename R>
 __device__ __forceinline__ cpx<R> sweep_rhs(const SweepConst<cpx<R>> &k, cpx<R> pn, cpx<R> pc, cpx<R> pp) {
@@ -347,7 +349,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
     __syncthreads();
     A x = sh.exl[l];
-    for (int t = P - 1; t > s; --t) x = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], x));
+    for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
     {
         auto fixb = [&](int j) { v[j] = m_fma(sh.gb[e0 + j], x, v[j]); };
         if (inM) fixb(M - 1);
@@ -373,12 +375,12 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     sh.exb[s * L + l] = sh.fa[top];  // the segment's product of calpha
     __syncthreads();
     A y = zero;  // psi'[0] = 0
-    for (int t = 0; t < s; ++t) y = m_add(sh.exa[t * L + l], m_mul(sh.exb[t * L + l], y));
+    for (int t = 0; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
     double acc = 0.0;
     {
         auto fixf = [&](int j) {
             v[j] = m_fma(sh.fa[e0 + j], y, v[j]);
-            if (COLS) acc += wz[e0 + j] * m_mod2(v[j]);
+            if (COLS) acc = fma(wz[e0 + j], m_mod2(v[j]), acc);
         };
         if (in0) fixf(0);
 #pragma unroll
@@ -432,14 +434,18 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
     }
 }
 
-// z sweep: lines are contiguous rows, so a workgroup of L rows is staged
-// through an LDS tile in the storage type (row-major, one element of padding:
-// coalesced row loads and stores, conflict-free segment reads), each thread
-// moves its segment between the tile and its registers (FP64 arithmetic).
-// With COLS (imaginary time) the Simpson z integrals of the new psi^2 are
-// summed per row and the rows of the tile with the y weights:
-// colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
-template <typename S, int L, int M, bool COLS>
+// z sweep: lines are contiguous rows.  A workgroup covers L rows of one x
+// plane.  Staged (DIRECT false): the rows go through an LDS tile in the
+// storage type (row-major, one element of padding: coalesced row loads and
+// stores, conflict-free segment reads), and each thread moves its segment
+// between the tile and its registers.  Direct (DIRECT true): each thread
+// loads its segment, M contiguous and 16-byte aligned values of its row,
+// straight into registers as packets and stores it back the same way; the
+// lanes of a wavefront read whole 64-128-byte runs of different rows, which
+// the caches merge.  With COLS (imaginary time) the Simpson z integrals of
+// the new psi^2 are summed per row and the rows of the tile with the y
+// weights: colpart[tile][x] (calcnorm, imag3d.cu:1194-1209).
+template <typename S, int L, int M, bool COLS, bool DIRECT>
 __global__ void __launch_bounds__(512)
 k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
           SweepConst<typename Arith<S>::T> k, const double *__restrict__ wz, const double *__restrict__ wy) {
@@ -461,40 +467,68 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     const int nl = min(L, Ny - y0);
     const long base = ((long)xrow * Ny + y0) * Nz;
     const A zero = m_zero(k.A);
-    // The tile's nl rows are one contiguous run of nl * n elements; each
-    // thread moves M / VE packets of it (all loads first).
     constexpr int VE = Pk<S>::n, KV = M / VE;
     static_assert(M % VE == 0, "z sweep packets");
     const int npk = nl * n / VE;
-    const CoefPre<A> cp = coef_pre(tab, n, tid);
-    typename Pk<S>::type r[KV];
-#pragma unroll
-    for (int q = 0; q < KV; ++q) {
-        const int iv = tid + q * nt;
-        r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
-    }
-    coef_put(sh, cp, tab, n, tid, nt);
-#pragma unroll
-    for (int q = 0; q < KV; ++q) {
-        const int iv = tid + q * nt;
-        if (iv < npk) {
-            const int e = (iv * VE) % n, row = (iv * VE) / n;
-#pragma unroll
-            for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
-        }
-    }
-    __syncthreads();
     const int e0 = s * M;
     const bool act = l < nl;
-    const S *row = tile + l * RS;
-    A v[M];
+    const CoefPre<A> cp = coef_pre(tab, n, tid);
+    A v[M], hl, hr;
+    if constexpr (DIRECT) {
+        // rows past the tile's last read that row (clamped) and are not stored
+        const S *src = psi + base + (long)min(l, nl - 1) * n + e0;
+        typename Pk<S>::type r[KV];
 #pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? to_a(row[e0 + j]) : zero;
-    const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
-    const A hr = (act && e0 + M < n) ? to_a(row[e0 + M]) : zero;
+        for (int q = 0; q < KV; ++q) r[q] = ld_pk(src + q * VE);
+        const S sl = src[e0 > 0 ? -1 : 0], sr_ = src[e0 + M < n ? M : M - 1];
+        coef_put(sh, cp, tab, n, tid, nt);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KV; ++q)
+#pragma unroll
+            for (int j = 0; j < VE; ++j) v[q * VE + j] = act ? to_a(r[q].e[j]) : zero;
+        hl = (act && e0 > 0) ? to_a(sl) : zero;
+        hr = (act && e0 + M < n) ? to_a(sr_) : zero;
+    } else {
+        // The tile's nl rows are one contiguous run of nl * n elements; each
+        // thread moves M / VE packets of it (all loads first).
+        typename Pk<S>::type r[KV];
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            const int iv = tid + q * nt;
+            r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
+        }
+        coef_put(sh, cp, tab, n, tid, nt);
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            const int iv = tid + q * nt;
+            if (iv < npk) {
+                const int e = (iv * VE) % n, row = (iv * VE) / n;
+#pragma unroll
+                for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
+            }
+        }
+        __syncthreads();
+        const S *row = tile + l * RS;
+#pragma unroll
+        for (int j = 0; j < M; ++j) v[j] = act ? to_a(row[e0 + j]) : zero;
+        hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
+        hr = (act && e0 + M < n) ? to_a(row[e0 + M]) : zero;
+    }
     const double acc = seg_solve<A, M, COLS, true>(v, hl, hr, e0, n, s, l, L, P, sh, k, wz);
     // every thread read its segment before the first barrier inside seg_solve
-    if (act) {
+    if constexpr (DIRECT) {
+        if (act) {
+            S *dst = psi + base + (long)l * n + e0;
+#pragma unroll
+            for (int q = 0; q < KV; ++q) {
+                typename Pk<S>::type o;
+#pragma unroll
+                for (int j = 0; j < VE; ++j) o.e[j] = from_a<S>(v[q * VE + j]);
+                st_pk(dst + q * VE, o);
+            }
+        }
+    } else if (act) {
         S *orow = tile + l * RS;
 #pragma unroll
         for (int j = 0; j < M; ++j) orow[e0 + j] = from_a<S>(v[j]);
@@ -512,18 +546,20 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
             w = wave_sum(w);
             if (tid == 0) colpart[(long)tno * Nx + xrow] = w;
         }
-    } else {
+    } else if constexpr (!DIRECT) {
         __syncthreads();
     }
+    if constexpr (!DIRECT) {
 #pragma unroll
-    for (int q = 0; q < KV; ++q) {
-        const int iv = tid + q * nt;
-        if (iv < npk) {
-            const int e = (iv * VE) % n, row = (iv * VE) / n;
-            typename Pk<S>::type o;
+        for (int q = 0; q < KV; ++q) {
+            const int iv = tid + q * nt;
+            if (iv < npk) {
+                const int e = (iv * VE) % n, row = (iv * VE) / n;
+                typename Pk<S>::type o;
 #pragma unroll
-            for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
-            st_pk(psi + base + (long)iv * VE, o);
+                for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
+                st_pk(psi + base + (long)iv * VE, o);
+            }
         }
     }
 }
@@ -642,35 +678,48 @@ static hipError_t sweep_xy_dispatch(const Launch &q, int Nx, int Ny, int Nz, con
     return sweep_xy_rows<S, AXIS, false>(q, Nx, Ny, Nz, t, k, rowb, M);
 }
 
-template <typename S, int L, bool COLS>
+template <typename S, int L, bool COLS, bool DIRECT>
 static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const SweepTab<typename Arith<S>::T> &t,
                                  const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
     using A = typename Arith<S>::T;
     const int P = Nz / 16;
     const long nblocks = (long)((Ny + L - 1) / L) * Nx;
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
-    const size_t lds = ((SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15)) + (size_t)(Nz + 1) * L * sizeof(S);
+    size_t lds = (SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15);
+    if (!DIRECT) lds += (size_t)(Nz + 1) * L * sizeof(S);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_z<S, L, 16, COLS>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_z<S, L, 16, COLS, DIRECT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    k_sweep_z<S, L, 16, COLS><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t, k, wz, wy);
+    k_sweep_z<S, L, 16, COLS, DIRECT><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t, k, wz, wy);
     return hipGetLastError();
 }
 
+template <typename S, bool COLS, bool DIRECT>
+static hipError_t sweep_z_rows(const Launch &q, int Nx, int Ny, int Nz, const SweepTab<typename Arith<S>::T> &t,
+                               const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
+    switch (z_rows<S>(Nz)) {
+    case 64: return sweep_z_launch<S, 64, COLS, DIRECT>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 32: return sweep_z_launch<S, 32, COLS, DIRECT>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 16: return sweep_z_launch<S, 16, COLS, DIRECT>(q, Nx, Ny, Nz, t, k, wz, wy);
+    case 8: return sweep_z_launch<S, 8, COLS, DIRECT>(q, Nx, Ny, Nz, t, k, wz, wy);
+    default: return sweep_z_launch<S, 4, COLS, DIRECT>(q, Nx, Ny, Nz, t, k, wz, wy);
+    }
+}
+
+// The z sweep staged through LDS or direct (DBEC_SWEEP_Z_DIRECT, else the
+// tuned choice, else staged).
 template <typename S, bool COLS>
 static hipError_t sweep_z_dispatch(const Launch &q, int Nx, int Ny, int Nz, const SweepTabs<typename Arith<S>::T> &ts,
-                                   const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy) {
+                                   const SweepConst<typename Arith<S>::T> &k, const double *wz, const double *wy,
+                                   const Tuning &tu) {
     if (Nz % 16 != 0) return hipErrorInvalidValue;
+    static const int direct_env = env_int("DBEC_SWEEP_Z_DIRECT", -1);
+    const int d = direct_env >= 0 ? direct_env : tu.z_direct;
     const SweepTab<typename Arith<S>::T> t = ts.at(16);
-    switch (z_rows<S>(Nz)) {
-    case 64: return sweep_z_launch<S, 64, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
-    case 32: return sweep_z_launch<S, 32, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
-    case 16: return sweep_z_launch<S, 16, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
-    case 8: return sweep_z_launch<S, 8, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
-    default: return sweep_z_launch<S, 4, COLS>(q, Nx, Ny, Nz, t, k, wz, wy);
-    }
+    if (d > 0) return sweep_z_rows<S, COLS, true>(q, Nx, Ny, Nz, t, k, wz, wy);
+    return sweep_z_rows<S, COLS, false>(q, Nx, Ny, Nz, t, k, wz, wy);
 }
 
 template <typename S, typename A>
@@ -684,8 +733,8 @@ static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const
     const double *wy = c->b.wgt + c->Nx, *wz = c->b.wgt + c->Nx + c->Ny;
     if (axis == 0) return sweep_xy_dispatch<S, 0>(q, Nx, Ny, Nz, t, k, c->tune);
     if (axis == 1) return sweep_xy_dispatch<S, 1>(q, Nx, Ny, Nz, t, k, c->tune);
-    if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, t, k, wz, wy);
-    return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, t, k, wz, wy);
+    if (cols) return sweep_z_dispatch<S, true>(q, Nx, Ny, Nz, t, k, wz, wy, c->tune);
+    return sweep_z_dispatch<S, false>(q, Nx, Ny, Nz, t, k, wz, wy, c->tune);
 }
 
 hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves &v, bool cols) {
