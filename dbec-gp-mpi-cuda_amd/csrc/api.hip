@@ -52,6 +52,22 @@ hipError_t launch_zinv(long n, const FftArgs &a) {
     default: return hipErrorInvalidValue;
     }
 }
+hipError_t launch_sweepz_zfwd(long n, const FftArgs &a) {
+    switch (n) {
+#define DBEC_X(NN) case NN: return FftLaunch<NN>::sweepz_zfwd(a);
+        DBEC_FOR_EACH_FFT_LEN(DBEC_X)
+#undef DBEC_X
+    default: return hipErrorInvalidValue;
+    }
+}
+bool sweepz_zfwd_fusable(long n, bool fp32, bool cpsi, int zrows) {
+    switch (n) {
+#define DBEC_X(NN) case NN: return FftLaunch<NN>::fusable(fp32, cpsi, zrows);
+        DBEC_FOR_EACH_FFT_LEN(DBEC_X)
+#undef DBEC_X
+    default: return false;
+    }
+}
 hipError_t launch_zinv_nu(long n, const FftArgs &a) {
     switch (n) {
 #define DBEC_X(NN) case NN: return FftLaunch<NN>::fused(a);
@@ -187,13 +203,15 @@ int run_calcpsidd2(dbec_ctx *c) {
     return DBEC_OK;
 }
 
-// The composite step's convolution: the forward passes, then the inverse z
-// pass fused with calcnu (interior rows) and the face rows.
-int run_conv_nu(dbec_ctx *c, const Solves &v) {
+// The composite step's convolution: the forward passes (the z pass unless
+// the previous step's fused z sweep did it: z_done), then the inverse z pass
+// fused with calcnu (interior rows) and the face rows.
+int run_conv_nu(dbec_ctx *c, const Solves &v, bool z_done) {
     const double nb = v.nb;
     const double ce = (double)(2 * real_elem(c));
     const double spec_b = (double)c->Nkz * c->Nx * c->Ny * ce;
-    HIPCHK(c, timed(c, "zfwd", nb * ((double)c->N * psi_elem(c) + spec_b), [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }), "zfwd");
+    if (!z_done)
+        HIPCHK(c, timed(c, "zfwd", nb * ((double)c->N * psi_elem(c) + spec_b), [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }), "zfwd");
     HIPCHK(c, timed(c, "ypass_fwd", nb * 2 * spec_b, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }), "ypass");
     const double pq = (double)c->Nkz * (c->Nx / 2 + 1) * (c->Ny / 2 + 1) * real_elem(c);
     HIPCHK(c, timed(c, "xconv", nb * (2 * spec_b + pq), [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }), "xconv");
@@ -212,15 +230,37 @@ int run_sweep(dbec_ctx *c, int axis, const Solves &v, bool cols) {
     return DBEC_OK;
 }
 
+// Whether the z sweep of a step can run fused with the next step's forward
+// z pass (the z length's plan has T = N / 8 and the kernel fits for the
+// precision and the z sweep's rows per workgroup; DBEC_FUSE_Z=0 turns it off).
+bool fuse_z(const dbec_ctx *c) {
+    static const bool on = [] {
+        const char *e = getenv("DBEC_FUSE_Z");
+        return !e || std::strcmp(e, "0") != 0;
+    }();
+    return on && sweepz_zfwd_fusable(c->Nz, c->prec == DBEC_MIXED, c->mode == DBEC_REAL, sweep_z_rows(c));
+}
+
 // One split step (calcpsidd2, calcnu, calclux/y/z) of the solves v.  In
 // imaginary time the z sweep leaves the Simpson column partials of the new
 // psi: the next step folds the norm in (v.fold), or run_norm computes it.
-int run_step(dbec_ctx *c, const Solves &v) {
+// z_done: this step's forward z pass already ran (fused into the previous
+// step's z sweep).  fuse_next: run this step's z sweep fused with the next
+// step's forward z pass, whose |psi|^2 it transforms unscaled (that step
+// then has v.fold == 2).
+int run_step(dbec_ctx *c, const Solves &v, bool z_done, bool fuse_next) {
     const bool imag = c->mode == DBEC_IMAG;
-    CHK(run_conv_nu(c, v));
+    CHK(run_conv_nu(c, v, z_done));
     CHK(run_sweep(c, 0, v, false));
     CHK(run_sweep(c, 1, v, false));
-    CHK(run_sweep(c, 2, v, imag));
+    if (!fuse_next) return run_sweep(c, 2, v, imag);
+    const double ce = (double)(2 * real_elem(c));
+    const double spec_b = (double)c->Nkz * c->Nx * c->Ny * ce;
+    const double bytes = v.nb * (2.0 * (double)c->N * psi_elem(c) + spec_b +
+                                 (imag ? (double)c->Nx * sweep_z_tiles(c) * 8.0 : 0.0));
+    FftArgs a = fft_args(c, 2, v);
+    sweep_z_args(c, a);
+    HIPCHK(c, timed(c, "sweep_z_zfwd", bytes, [&] { return launch_sweepz_zfwd(c->Nz, a); }), "sweep_z_zfwd");
     return DBEC_OK;
 }
 
@@ -815,6 +855,8 @@ int dbec_step(dbec_ctx *c, long nsteps, double g_stp, double gd_stp, double *nor
     if (!c || nsteps < 0) return fail(c, DBEC_EINVAL, "bad argument");
     (void)hipSetDevice(c->device);
     const bool imag = c->mode == DBEC_IMAG;
+    const bool fuse = fuse_z(c);
+    bool z_done = false;
     for (long s = 0; s < nsteps; s++) {
         if (g_stp != 0. || gd_stp != 0.) {
             c->g += g_stp;
@@ -822,8 +864,10 @@ int dbec_step(dbec_ctx *c, long nsteps, double g_stp, double gd_stp, double *nor
         }
         // after the first step the previous norm is folded into this one
         Solves v = solve_ref(c, imag && (c->scale_pending || s > 0));
-        v.fold = (imag && s > 0) ? 1 : 0;
-        CHK(run_step(c, v));
+        v.fold = (imag && s > 0) ? (z_done ? 2 : 1) : 0;
+        const bool fuse_next = fuse && s + 1 < nsteps;
+        CHK(run_step(c, v, z_done, fuse_next));
+        z_done = fuse_next;
     }
     if (imag && nsteps > 0) {
         CHK(run_norm(c, solve_ref(c, false)));
@@ -1023,7 +1067,10 @@ namespace dbec {
 
 Solves core_solves(const dbec_ctx *c, bool use_scale) { return solve_ref(c, use_scale); }
 
-int core_step(dbec_ctx *c, const Solves &v) { return run_step(c, v); }
+int core_step(dbec_ctx *c, const Solves &v, bool z_done, bool fuse_next) {
+    return run_step(c, v, z_done, fuse_next);
+}
+bool core_fuse_z(const dbec_ctx *c) { return fuse_z(c); }
 
 int core_norm(dbec_ctx *c, const Solves &v) { return run_norm(c, v); }
 

@@ -13,8 +13,12 @@ namespace dbec {
 Solves core_solves(const dbec_ctx *c, bool use_scale);
 // One split step of the solves v; c supplies grid, tables and stream.  In
 // imaginary time the step leaves the norm in the column partials: set v.fold
-// on the next step, or call core_norm.
-int core_step(dbec_ctx *c, const Solves &v);
+// on the next step (2 when this step ran with fuse_next, else 1), or call
+// core_norm.  z_done: the previous step ran with fuse_next (its z sweep did
+// this step's forward z pass); fuse_next only where core_fuse_z(c) and
+// another step follows.
+int core_step(dbec_ctx *c, const Solves &v, bool z_done, bool fuse_next);
+bool core_fuse_z(const dbec_ctx *c);
 int core_norm(dbec_ctx *c, const Solves &v);
 // Observables of c's psi, asynchronous on c->stream, results in c->b.scal.
 // Imaginary time first applies the pending 1/norm (scal[SC_SCALE] -> psi,

@@ -21,6 +21,7 @@
 #include "fft_launch.hpp"
 #include "internal.hpp"
 #include "norm_device.hpp"
+#include "sweep_device.hpp"
 
 namespace dbec {
 
@@ -134,16 +135,190 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
             spec[((long)k * Nx + x) * Ny + y0 + row] = o;
         }
     }
-    if (v.fold && blockIdx.x == 0) {  // the last step's norm, for the x convolution and calcnu
+    if (v.fold == 1 && blockIdx.x == 0) {  // the last step's norm, for the x convolution and calcnu
         __syncthreads();
         norm_from_cols<WG>(sr.rows, tiles, wx, Nx, sr.scal, v.shift, reinterpret_cast<double *>(buf));
     }
 }
 
 // ---------------------------------------------------------------------------
+// The z sweep of one step fused with the forward z pass of the next
+// (calcluz, imag3d.cu:1895-1942, then the first transform of calcpsidd2):
+// both walk the same contiguous z rows.  A workgroup takes L rows of one x
+// plane through the LDS tile of k_sweep_z (partitioned Thomas solve, Simpson
+// column partials with COLS), stores psi, then reuses the tile's LDS for the
+// |psi|^2 of its rows, two rows per complex line, and runs the forward
+// transform and split of k_zfwd on them, so psi is read once for both.  The
+// rows transform unscaled: the next step's norm comes from the column
+// partials (v.fold == 2, first workgroup of the forward y pass).  Built for
+// plans with T = N / 8, where L / 2 lines of T threads are the sweep's L * P
+// threads (P = N / 16 segments per row).
+template <int N>
+struct FusedZ {
+    static constexpr bool ok = FftPlan<N>::T * 8 == N;
+    // LDS of the fused kernel: sweep tables and exchange, the tile / transform
+    // buffer, the twiddles
+    // The tile and the transform buffer share one region; the row partials
+    // of the z integrals sit in its part past the tile when that is free
+    // (they are summed before the buffer is filled), else with the sweep's
+    // exchange.
+    template <typename S, typename R, int L>
+    static constexpr size_t tile_b() { return ((size_t)(N + 1) * L * sizeof(S) + 15) & ~size_t(15); }
+    template <typename S, typename R, int L>
+    static constexpr size_t x_b() {
+        constexpr size_t tile = tile_b<S, R, L>(), buf = (size_t)N * (L / 2 + 1) * sizeof(cpx<R>);
+        return ((tile > buf ? tile : buf) + 15) & ~size_t(15);
+    }
+    template <typename S, typename R, int L>
+    static constexpr bool exn_in_x() {
+        return x_b<S, R, L>() - tile_b<S, R, L>() >= (size_t)L * (N / 16) * sizeof(double);
+    }
+    template <typename S, typename R, int L>
+    static constexpr size_t sh_b() {
+        using A = typename Arith<S>::T;
+        return (SweepShared<A>::bytes(N, L * (N / 16), L, !exn_in_x<S, R, L>()) + 15) & ~size_t(15);
+    }
+    template <typename S, typename R, int L>
+    static constexpr size_t lds() {
+        return sh_b<S, R, L>() + x_b<S, R, L>() + N * sizeof(cpx<R>);
+    }
+    // the kernel exists for rows L: whole transform lines, at most 512
+    // threads, LDS within 160 KiB
+    template <typename S, typename R, int L>
+    static constexpr bool fits() {
+        return ok && (L / 2) * FftPlan<N>::T == L * (N / 16) && L * (N / 16) <= 512 && lds<S, R, L>() <= 160 * 1024;
+    }
+};
+
+template <int N, typename S, typename R, int L, bool COLS>
+__global__ void __launch_bounds__(L *(N / 16))
+k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, SweepConst<typename Arith<S>::T> k,
+              const double *__restrict__ wz, const double *__restrict__ wy, const cpx<R> *__restrict__ twg) {
+    using A = typename Arith<S>::T;
+    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, LS = LF + 1, RS = N + 1;
+    static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
+    using F = FusedZ<N>;
+    constexpr size_t SH_B = F::template sh_b<S, R, L>(), TILE_B = F::template tile_b<S, R, L>();
+    constexpr size_t X_B = F::template x_b<S, R, L>();
+    static_assert(TILE_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * LS * sizeof(cpx<R>), "fused z LDS");
+    __shared__ __align__(16) char smem[F::template lds<S, R, L>()];
+    const SolveAt sr = solve_at(v);
+    S *__restrict__ psi = reinterpret_cast<S *>(sr.psi);
+    cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(sr.spec);
+    SweepShared<A> sh(smem, N, WG, L,
+                      F::template exn_in_x<S, R, L>() ? reinterpret_cast<double *>(smem + SH_B + TILE_B) : nullptr);
+    S *tile = reinterpret_cast<S *>(smem + SH_B);
+    cpx<R> *buf = reinterpret_cast<cpx<R> *>(smem + SH_B);  // after the tile is stored
+    cpx<R> *tw = reinterpret_cast<cpx<R> *>(smem + SH_B + X_B);
+    const int tid = threadIdx.x;
+    const int l = tid % L, s = tid / L;
+    const int yt = (Ny + L - 1) / L;
+    const int xrow = blockIdx.x / yt, tno = blockIdx.x % yt, y0 = tno * L;
+    const int nl = min(L, Ny - y0);
+    const long base = ((long)xrow * Ny + y0) * N;
+    const A zero = m_zero(k.A);
+    constexpr int VE = Pk<S>::n, KV = M / VE;
+    const int npk = nl * N / VE;
+    const CoefPre<A> cp = coef_pre(tab, N, tid);
+    {
+        typename Pk<S>::type r[KV];
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            const int iv = tid + q * WG;
+            r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
+        }
+        coef_put(sh, cp, tab, N, tid, WG);
+        stage_twiddles(tw, twg, N, tid, WG);
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            const int iv = tid + q * WG;
+            if (iv < npk) {
+                const int e = (iv * VE) % N, row = (iv * VE) / N;
+#pragma unroll
+                for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
+            }
+        }
+    }
+    __syncthreads();
+    const int e0 = s * M;
+    const bool act = l < nl;
+    A vv[M];
+    {
+        const S *row = tile + l * RS;
+#pragma unroll
+        for (int j = 0; j < M; ++j) vv[j] = act ? to_a(row[e0 + j]) : zero;
+        const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
+        const A hr = (act && e0 + M < N) ? to_a(row[e0 + M]) : zero;
+        const double acc = seg_solve<A, M, COLS, true>(vv, hl, hr, e0, N, s, l, L, P, sh, k, wz);
+        if (act) {
+            S *orow = tile + l * RS;
+#pragma unroll
+            for (int j = 0; j < M; ++j) orow[e0 + j] = from_a<S>(vv[j]);
+        }
+        if constexpr (COLS) {
+            sh.exn[s * L + l] = acc;
+            __syncthreads();
+            if (tid < 64) {
+                double w = 0.0;
+                if (tid < L && tid < nl) {
+                    double rsum = 0.0;
+                    for (int t = 0; t < P; ++t) rsum += sh.exn[t * L + tid];
+                    w = wy[y0 + tid] * rsum;
+                }
+                w = wave_sum(w);
+                if (tid == 0) sr.rows[(long)tno * Nx + xrow] = w;
+            }
+        } else {
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KV; ++q) {
+        const int iv = tid + q * WG;
+        if (iv < npk) {
+            const int e = (iv * VE) % N, row = (iv * VE) / N;
+            typename Pk<S>::type o;
+#pragma unroll
+            for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
+            st_pk(psi + base + (long)iv * VE, o);
+        }
+    }
+    __syncthreads();  // the tile is stored; its LDS becomes the transform buffer
+    // |psi|^2 of the new rows (zero past the plane's last row): row 2p is the
+    // real part, row 2p + 1 the imaginary part of line p
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const R m2 = act ? (R)PsiTraits<S>::mod2(from_a<S>(vv[j]), 1.0) : R(0);
+        R *slot = reinterpret_cast<R *>(&buf[(e0 + j) * LS + (l >> 1)]);
+        slot[l & 1] = m2;
+    }
+    __syncthreads();
+    FftPlan<N>::template run<-1>(buf, LS, tid % LF, tid / LF, tw);
+    // split Z = A + iB into the spectra of the two real rows (as k_zfwd)
+    constexpr int nkz = N / 2 + 1;
+#pragma unroll
+    for (int q = 0; q < (nkz * L + WG - 1) / WG; ++q) {
+        const int i = tid + q * WG;
+        const int row = i % L, kz = i / L;
+        if (kz < nkz && row < nl) {
+            const int p = row >> 1;
+            const cpx<R> z = buf[kz * LS + p];
+            const cpx<R> w = buf[((N - kz) % N) * LS + p];
+            cpx<R> o;
+            if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
+            else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
+            spec[((long)kz * Nx + xrow) * Ny + y0 + row] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// With v.fold == 2 (the forward z pass ran fused with the last z sweep) the
+// first workgroup of each solve's forward pass also forms the last step's
+// norm from the z sweep's column partials, for the x convolution and calcnu.
 template <int N, int DIR, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
-k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg) {
+k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg, const double *__restrict__ wx, int tiles, int Nx) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
@@ -172,6 +347,11 @@ k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg) {
     for (int k = 0; k < K; ++k) {
         const int i = tid + k * WG;
         if (i < nl * N) g[i] = buf[(i % N) * LS + i / N];
+    }
+    if (DIR < 0 && v.fold == 2 && blockIdx.x == 0) {
+        const SolveAt sr = solve_at(v);
+        __syncthreads();  // buf becomes the reduction scratch
+        norm_from_cols<WG>(sr.rows, tiles, wx, Nx, sr.scal, v.shift, reinterpret_cast<double *>(buf));
     }
 }
 
@@ -526,12 +706,12 @@ struct FftLaunchImpl {
         dim3 grid((unsigned)((nlines + Tl::L - 1) / Tl::L), (unsigned)a.v.nb), block(Tl::WG);
         if (!a.fp32) {
             auto *tw = static_cast<const cpx<double> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
-            else k_ypass<N, +1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw, a.wx, a.tiles, (int)a.Nx);
+            else k_ypass<N, +1, double, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw, a.wx, a.tiles, (int)a.Nx);
         } else {
             auto *tw = static_cast<const cpx<float> *>(a.tw);
-            if (dir < 0) k_ypass<N, -1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
-            else k_ypass<N, +1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw);
+            if (dir < 0) k_ypass<N, -1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw, a.wx, a.tiles, (int)a.Nx);
+            else k_ypass<N, +1, float, LM><<<grid, block, 0, a.stream>>>(a.v, nlines, tw, a.wx, a.tiles, (int)a.Nx);
         }
         return hipGetLastError();
     }
@@ -556,6 +736,78 @@ struct FftLaunchImpl {
     static hipError_t xconv(const FftArgs &a) {
         static const int l64 = fft_lm("DBEC_FFT_LM_XCONV64", 0), l32 = fft_lm("DBEC_FFT_LM_XCONV32", 0);
         return pick_lm(a, a.fp32 ? l32 : l64, 2, a.fp32 ? 2 : 1) == 2 ? xconv_t<2>(a) : xconv_t<1>(a);
+    }
+
+    template <typename S, typename R, bool COLS, int L>
+    static hipError_t sweepz_zfwd_t(const FftArgs &a) {
+        using A = typename Arith<S>::T;
+        if constexpr (!FusedZ<N>::template fits<S, R, L>()) {
+            return hipErrorInvalidValue;
+        } else {
+            const int yt = (int)((a.Ny + L - 1) / L);
+            dim3 grid((unsigned)(a.Nx * yt), (unsigned)a.v.nb), block(L * (N / 16));
+            const SweepTab<A> tab{static_cast<const A *>(a.ztab[0]), static_cast<const A *>(a.ztab[1]),
+                                  static_cast<const A *>(a.ztab[2]), static_cast<const A *>(a.ztab[3])};
+            SweepConst<A> k;
+            zconst(a, k);
+            k_sweepz_zfwd<N, S, R, L, COLS><<<grid, block, 0, a.stream>>>(a.v, (int)a.Nx, (int)a.Ny, tab, k, a.wz, a.wy,
+                                                                         static_cast<const cpx<R> *>(a.tw));
+            return hipGetLastError();
+        }
+    }
+    static void zconst(const FftArgs &a, SweepConst<double> &k) {
+        k.A0r = a.zk[0][0];
+        k.A = a.zk[1][0];
+        k.mA = a.zk[2][0];
+    }
+    static void zconst(const FftArgs &a, SweepConst<float> &k) {
+        k.A0r = (float)a.zk[0][0];
+        k.A = (float)a.zk[1][0];
+        k.mA = (float)a.zk[2][0];
+    }
+    static void zconst(const FftArgs &a, SweepConst<cpx<double>> &k) {
+        k.A0r = {a.zk[0][0], a.zk[0][1]};
+        k.A = {a.zk[1][0], a.zk[1][1]};
+        k.mA = {a.zk[2][0], a.zk[2][1]};
+    }
+    static void zconst(const FftArgs &a, SweepConst<cpx<float>> &k) {
+        k.A0r = {(float)a.zk[0][0], (float)a.zk[0][1]};
+        k.A = {(float)a.zk[1][0], (float)a.zk[1][1]};
+        k.mA = {(float)a.zk[2][0], (float)a.zk[2][1]};
+    }
+    template <typename S, typename R, bool COLS>
+    static hipError_t sweepz_zfwd_rows(const FftArgs &a) {
+        switch (a.zrows) {
+        case 64: return sweepz_zfwd_t<S, R, COLS, 64>(a);
+        case 32: return sweepz_zfwd_t<S, R, COLS, 32>(a);
+        case 16: return sweepz_zfwd_t<S, R, COLS, 16>(a);
+        case 8: return sweepz_zfwd_t<S, R, COLS, 8>(a);
+        case 4: return sweepz_zfwd_t<S, R, COLS, 4>(a);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    static hipError_t sweepz_zfwd(const FftArgs &a) {
+        if (!a.fp32) {
+            if (a.cpsi) return sweepz_zfwd_rows<cpx<double>, double, false>(a);
+            return sweepz_zfwd_rows<double, double, true>(a);
+        }
+        if (a.cpsi) return sweepz_zfwd_rows<cpx<float>, float, false>(a);
+        return sweepz_zfwd_rows<float, float, true>(a);
+    }
+    template <typename S, typename R>
+    static bool fits_rows(int zrows) {
+        switch (zrows) {
+        case 64: return FusedZ<N>::template fits<S, R, 64>();
+        case 32: return FusedZ<N>::template fits<S, R, 32>();
+        case 16: return FusedZ<N>::template fits<S, R, 16>();
+        case 8: return FusedZ<N>::template fits<S, R, 8>();
+        case 4: return FusedZ<N>::template fits<S, R, 4>();
+        default: return false;
+        }
+    }
+    static bool fusable(bool fp32, bool cpsi, int zrows) {
+        if (!fp32) return cpsi ? fits_rows<cpx<double>, double>(zrows) : fits_rows<double, double>(zrows);
+        return cpsi ? fits_rows<cpx<float>, float>(zrows) : fits_rows<float, float>(zrows);
     }
 
     // single solve: spectrum of `one` -> phi
@@ -608,5 +860,9 @@ template <int N>
 hipError_t FftLaunch<N>::zinv(const FftArgs &a) { return FftLaunchImpl<N>::zinv(a); }
 template <int N>
 hipError_t FftLaunch<N>::fused(const FftArgs &a) { return FftLaunchImpl<N>::fused(a); }
+template <int N>
+hipError_t FftLaunch<N>::sweepz_zfwd(const FftArgs &a) { return FftLaunchImpl<N>::sweepz_zfwd(a); }
+template <int N>
+bool FftLaunch<N>::fusable(bool fp32, bool cpsi, int zrows) { return FftLaunchImpl<N>::fusable(fp32, cpsi, zrows); }
 
 }  // namespace dbec

@@ -15,6 +15,10 @@ struct FftLaunch {
     static hipError_t xconv(const FftArgs &a);
     static hipError_t zinv(const FftArgs &a);   // single solve: spectrum -> phi
     static hipError_t fused(const FftArgs &a);  // zinv fused with calcnu
+    // the z sweep fused with the next step's forward z pass, for the
+    // precisions and z sweep rows per workgroup where fusable() holds
+    static hipError_t sweepz_zfwd(const FftArgs &a);
+    static bool fusable(bool fp32, bool cpsi, int zrows);
 };
 
 }  // namespace dbec

@@ -148,6 +148,13 @@ struct FftArgs {
     const double *wx;     // Simpson weights along x (folded norm)
     int tiles;            // rows of the column partials (folded norm)
     Tuning tune;          // the context's launch choices
+    // z sweep fused with the forward z pass: tables (calpha, cgamma, gb, fa of
+    // the z axis, segments of 16), constants A0r, A, mA (re, im), Simpson
+    // weights along z and y, rows per workgroup
+    const void *ztab[4];
+    double zk[3][2];
+    const double *wz, *wy;
+    int zrows;
 };
 // Each returns hipSuccess or an error; `n` is the transformed length.
 hipError_t launch_zfwd(long n, const FftArgs &a);    // |psi|^2 -> spec (R2C along z)
@@ -158,5 +165,9 @@ hipError_t launch_zinv(long n, const FftArgs &a);    // spec -> phi (C2R along z
 // the face rows x = Nx-1 or y = Ny-1 with the reference's periodic face
 // copies of phi, in one launch.
 hipError_t launch_zinv_nu(long n, const FftArgs &a);
+// The z sweep of one step fused with the forward z pass of the next; whether
+// length n has it for the precision and z sweep rows per workgroup.
+hipError_t launch_sweepz_zfwd(long n, const FftArgs &a);
+bool sweepz_zfwd_fusable(long n, bool fp32, bool cpsi, int zrows);
 
 }  // namespace dbec

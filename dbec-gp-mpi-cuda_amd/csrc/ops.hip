@@ -8,6 +8,7 @@
 
 #include "norm_device.hpp"
 #include "ops.hpp"
+#include "sweep_device.hpp"
 
 namespace dbec {
 
@@ -147,252 +148,6 @@ hipError_t launch_set_scalar(hipStream_t s, double *p, double v) {
 }
 
 // ---------------------------------------------------------------------------
-// Thomas sweeps (calclux/calcluy/calcluz kernels, imag3d.cu:1802-1942;
-// real3d.cu:1902-2040), as a partitioned solve.  Every line is split into
-// P = n / M segments of M elements held in the registers of one thread.
-//
-// Backward (reference: cbeta[N-2] = psi[N-1]; cbeta[i-1] = cgamma[i] (A
-// cbeta[i] - c_i), c_i = minusA psi[i+1] + A0r psi[i] - A psi[i-1]): each
-// segment runs the same recurrence from a zero carry, Q, and the product of
-// its coefficients cgamma[m] A, Pb; then cbeta[i-1] = Q[i-1] + Pb[i-1] x with
-// x = cbeta at the segment's top, handed over from the segment on the right
-// (psi[N-1] for the last one).  Forward (psi'[0] = 0; psi'[i+1] = calpha[i]
-// psi'[i] + cbeta[i]; psi'[N-1] = 0): the same with the carry psi' coming
-// from the left.  Carries are folded through a small LDS exchange.  The
-// algebra is exact; results differ from the sequential sweep by round-off
-// only.  The coefficient tables sit in LDS so no step of the dependent
-// chains waits on a cache.  cbeta[i-1] replaces psi[i] in its register.
-__device__ __forceinline__ double m_mul(double a, double b) { return a * b; }
-__device__ __forceinline__ double m_add(double a, double b) { return a + b; }
-__device__ __forceinline__ double m_sub(double a, double b) { return a - b; }
-__device__ __forceinline__ double m_fma(double a, double b, double c) { return fma(a, b, c); }
-__device__ __forceinline__ double m_zero(double) { return 0.0; }
-__device__ __forceinline__ double m_mod2(double a) { return a * a; }
-__device__ __forceinline__ cpx<double> m_mul(cpx<double> a, cpx<double> b) { return cmul(a, b); }
-__device__ __forceinline__ cpx<double> m_add(cpx<double> a, cpx<double> b) { return a + b; }
-__device__ __forceinline__ cpx<double> m_sub(cpx<double> a, cpx<double> b) { return a - b; }
-__device__ __forceinline__ cpx<double> m_fma(cpx<double> a, cpx<double> b, cpx<double> c) {
-    double re = fma(a.x, b.x, c.x);
-    re = fma(-a.y, b.y, re);
-    double im = fma(a.x, b.y, c.y);
-    im = fma(a.y, b.x, im);
-    return {re, im};
-}
-__device__ __forceinline__ cpx<double> m_zero(cpx<double>) { return {0.0, 0.0}; }
-__device__ __forceinline__ double m_mod2(cpx<double> a) { return a.x * a.x + a.y * a.y; }
-// FP32 (mixed form): the recurrences in single precision, the Simpson
-// partials of |psi|^2 in double.
-__device__ __forceinline__ float m_mul(float a, float b) { return a * b; }
-__device__ __forceinline__ float m_add(float a, float b) { return a + b; }
-__device__ __forceinline__ float m_sub(float a, float b) { return a - b; }
-__device__ __forceinline__ float m_fma(float a, float b, float c) { return fmaf(a, b, c); }
-__device__ __forceinline__ float m_zero(float) { return 0.0f; }
-__device__ __forceinline__ double m_mod2(float a) { return (double)a * (double)a; }
-__device__ __forceinline__ cpx<float> m_mul(cpx<float> a, cpx<float> b) { return cmul(a, b); }
-__device__ __forceinline__ cpx<float> m_add(cpx<float> a, cpx<float> b) { return a + b; }
-__device__ __forceinline__ cpx<float> m_sub(cpx<float> a, cpx<float> b) { return a - b; }
-__device__ __forceinline__ cpx<float> m_fma(cpx<float> a, cpx<float> b, cpx<float> c) {
-    float re = fmaf(a.x, b.x, c.x);
-    re = fmaf(-a.y, b.y, re);
-    float im = fmaf(a.x, b.y, c.y);
-    im = fmaf(a.y, b.x, im);
-    return {re, im};
-}
-__device__ __forceinline__ cpx<float> m_zero(cpx<float>) { return {0.0f, 0.0f}; }
-__device__ __forceinline__ double m_mod2(cpx<float> a) { return (double)a.x * a.x + (double)a.y * a.y; }
-
-// Storage and arithmetic types: FP64 lines in FP64, FP32 lines (mixed form)
-// in FP32 with FP32 coefficient tables rounded from the FP64 ones.
-template <typename S> struct Arith { using T = S; };
-template <typename S> __device__ __forceinline__ S to_a(S v) { return v; }
-template <typename S> __device__ __forceinline__ S from_a(S v) { return v; }
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Device coefficient tables of one axis: calpha and cgamma (n - 1 each), and
-// for the segment length M of the launch the carry multipliers of the
-// partitioned solve (n each):
-//   gb[i] = prod_{t = i .. top} cgamma[t] A,   fa[e] = prod_{t = bot .. e} calpha[t - 1]
-// over the in-range indices (1..n-2) of the segment holding i / e, top and
-// bot its last and first in-range index.  They are products of coefficients
-// only, formed on the host in FP64.
-template <typename A>
-struct SweepTab {
-    const A *ca, *cg, *gb, *fa;
-};
-
-// LDS layout shared by both sweep kernels: the four tables (n each), the
-// carry exchange (one per thread), the last elements, the row partials.
-template <typename A>
-struct SweepShared {
-    A *ca, *cg, *gb, *fa, *exa, *exb, *exl;
-    double *exn;
-    __device__ SweepShared(char *raw, int n, int nthreads, int L) {
-        ca = reinterpret_cast<A *>(raw);
-        cg = ca + n;
-        gb = cg + n;
-        fa = gb + n;
-        exa = fa + n;
-        exb = exa + nthreads;
-        exl = exb + nthreads;
-        exn = reinterpret_cast<double *>(exl + L);
-    }
-    __host__ __device__ static size_t bytes(int n, int nthreads, int L) {
-        return (4 * (size_t)n + 2 * (size_t)nthreads + L) * sizeof(A) + (size_t)nthreads * sizeof(double);
-    }
-};
-
-// The tables are staged into LDS in two parts: the first slot of each thread
-// is loaded (coef_pre) before the thread's psi loads, so writing it to LDS
-// (coef_put) waits only for that older load while the psi loads are still
-// in flight; slots beyond the first thread count follow.
-template <typename A>
-struct CoefPre {
-    A ca, cg, gb, fa;
-};
-template <typename A>
-__device__ __forceinline__ CoefPre<A> coef_pre(const SweepTab<A> &t, int n, int tid) {
-    const int i = tid < n ? tid : 0, ic = i < n - 1 ? i : 0;
-    return {t.ca[ic], t.cg[ic], t.gb[i], t.fa[i]};
-}
-template <typename A>
-__device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const SweepTab<A> &t, int n,
-                                         int tid, int nthreads) {
-    if (tid < n) {
-        if (tid < n - 1) {
-            sh.ca[tid] = c.ca;
-            sh.cg[tid] = c.cg;
-        }
-        sh.gb[tid] = c.gb;
-        sh.fa[tid] = c.fa;
-    }
-    for (int i = tid + nthreads; i < n; i += nthreads) {
-        if (i < n - 1) {
-            sh.ca[i] = t.ca[i];
-            sh.cg[i] = t.cg[i];
-        }
-        sh.gb[i] = t.gb[i];
-        sh.fa[i] = t.fa[i];
-    }
-}
-
-// The backward recurrence's right-hand side c_i = mA psi[i+1] + A0r psi[i] -
-// A psi[i-1] and update cbeta[i-1] = cgamma[i] (A cbeta[i] - c_i).  Complex
-// lines (real time) have A = i a, mA = -A and A0r = 1 + i b (gencoef,
-// real3d.cu:991-1035; launch_sweep checks it), so there
-//   c = -i a (psi[i+1] + psi[i-1]) + (1 + i b) psi[i],  A cbeta - c = i a cbeta - c.
-// Real lines (imaginary time): mA = -A exactly, so c = mA (psi[i+1] +
-// psi[i-1]) + A0r psi[i]; both forms are written as explicit FMAs.
-template <typename T>
-__device__ __forceinline__ T sweep_rhs(const SweepConst<T> &k, T pn, T pc, T pp) {
-    return fma(k.mA, pn + pp, k.A0r * pc);
-}
-template <typename T>
-__device__ __forceinline__ T sweep_back(const SweepConst<T> &k, T cg, T bq, T c) {
-    return cg * fma(k.A, bq, -c);
-}
-template <typename R>
-__device__ __forceinline__ cpx<R> sweep_rhs(const SweepConst<cpx<R>> &k, cpx<R> pn, cpx<R> pc, cpx<R> pp) {
-    const R a = k.A.y, b = k.A0r.y;
-    const R sx = pn.x + pp.x, sy = pn.y + pp.y;
-    return {fma(a, sy, fma(-b, pc.y, pc.x)), fma(-a, sx, fma(b, pc.x, pc.y))};
-}
-template <typename R>
-__device__ __forceinline__ cpx<R> sweep_back(const SweepConst<cpx<R>> &k, cpx<R> cg, cpx<R> bq, cpx<R> c) {
-    const R a = k.A.y;
-    return cmul(cg, cpx<R>{fma(-a, bq.y, -c.x), fma(a, bq.x, -c.y)});
-}
-
-// The partitioned solve of one segment (registers v, halos hl / hr).  Every
-// thread of the workgroup calls it.  Returns, with COLS, the Simpson z
-// partial sum_j wz[e] |psi'|^2 of the segment.  Of the line's elements only
-// 0 (first segment, j = 0) and n-1 (last segment, j = M-1) fall outside the
-// recurrences' range 1..n-2, so with PEEL each pass tests only its first and
-// last iteration and runs the M-2 between them unconditionally; without it
-// every iteration keeps the test.  The x / y sweeps of complex lines use the
-// latter: peeled, the compiler schedules them with more registers, which
-// measured slower there (and faster in every other sweep).
-template <typename A, int M, bool COLS, bool PEEL>
-__device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n, int s, int l, int L, int P,
-                                            const SweepShared<A> &sh, const SweepConst<A> &k,
-                                            const double *__restrict__ wz) {
-    static_assert(M >= 2, "segment length");
-    const bool in0 = s > 0, inM = s < P - 1;  // j = 0 / j = M-1 inside 1..n-2
-    auto inr = [&](int j) { return PEEL || (e0 + j >= 1 && e0 + j <= n - 2); };  // middle iterations
-    const int bot = e0 > 1 ? e0 : 1, top = e0 + M - 1 < n - 2 ? e0 + M - 1 : n - 2;
-    const A zero = m_zero(k.A);
-    A bq = zero;
-    {
-        A pn = hr, pc = v[M - 1];
-        auto back = [&](int j, A pp) {
-            bq = sweep_back(k, sh.cg[e0 + j], bq, sweep_rhs(k, pn, pc, pp));
-            v[j] = bq;
-        };
-        if (inM) back(M - 1, v[M - 2]);
-        pn = pc;
-        pc = v[M - 2];
-#pragma unroll
-        for (int j = M - 2; j >= 1; --j) {
-            const A pp = v[j - 1];
-            if (inr(j)) back(j, pp);
-            pn = pc;
-            pc = pp;
-        }
-        if (in0) back(0, hl);
-    }
-    sh.exa[s * L + l] = bq;
-    sh.exb[s * L + l] = sh.gb[bot];  // the segment's product of cgamma A
-    if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
-    __syncthreads();
-    A x = sh.exl[l];
-    for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
-    {
-        auto fixb = [&](int j) { v[j] = m_fma(sh.gb[e0 + j], x, v[j]); };
-        if (inM) fixb(M - 1);
-#pragma unroll
-        for (int j = M - 2; j >= 1; --j)
-            if (inr(j)) fixb(j);
-        if (in0) fixb(0);
-    }
-    __syncthreads();  // carries read; the exchange is reused below
-    A fy = zero;
-    {
-        auto fwd = [&](int j) {
-            fy = m_fma(sh.ca[e0 + j - 1], fy, v[j]);
-            v[j] = fy;
-        };
-        if (in0) fwd(0);
-#pragma unroll
-        for (int j = 1; j <= M - 2; ++j)
-            if (inr(j)) fwd(j);
-        if (inM) fwd(M - 1);
-    }
-    sh.exa[s * L + l] = fy;
-    sh.exb[s * L + l] = sh.fa[top];  // the segment's product of calpha
-    __syncthreads();
-    A y = zero;  // psi'[0] = 0
-    for (int t = 0; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
-    double acc = 0.0;
-    {
-        auto fixf = [&](int j) {
-            v[j] = m_fma(sh.fa[e0 + j], y, v[j]);
-            if (COLS) acc = fma(wz[e0 + j], m_mod2(v[j]), acc);
-        };
-        if (in0) fixf(0);
-#pragma unroll
-        for (int j = 1; j <= M - 2; ++j)
-            if (inr(j)) fixf(j);
-        if (inM) fixf(M - 1);
-    }
-    if (s == 0) v[0] = zero;
-    if (s == P - 1) v[M - 1] = zero;
-    return acc;
-}
-
 // x and y sweeps: a workgroup covers L consecutive z (lanes; one 256-byte row
 // per load) of the lines of one y (x sweep) or x (y sweep) plane; thread
 // (l, s) loads its segment straight from HBM into registers.
@@ -600,6 +355,31 @@ static int z_rows(long n) {
 }
 
 static size_t arith_elem(const dbec_ctx *c) { return c->mode == DBEC_REAL ? 16 : 8; }
+
+int sweep_z_rows(const dbec_ctx *c) {
+    const bool f32 = c->prec == DBEC_MIXED;
+    if (c->mode == DBEC_IMAG) return f32 ? z_rows<float>(c->Nz) : z_rows<double>(c->Nz);
+    return f32 ? z_rows<cpx<float>>(c->Nz) : z_rows<cpx<double>>(c->Nz);
+}
+
+void sweep_z_args(const dbec_ctx *c, FftArgs &a) {
+    const CoefLayout lay(c->Nx, c->Ny, c->Nz);
+    const bool f32 = c->prec == DBEC_MIXED, cplx = c->mode == DBEC_REAL;
+    const size_t el = (f32 ? sizeof(float) : sizeof(double)) * (cplx ? 2 : 1);
+    const char *base = static_cast<const char *>(f32 ? c->b.coef_lo : static_cast<const void *>(c->b.coef));
+    a.ztab[0] = base + el * lay.ca(2);
+    a.ztab[1] = base + el * lay.cg(2);
+    a.ztab[2] = base + el * lay.gb(2, 0);
+    a.ztab[3] = base + el * lay.fa(2, 0);
+    for (int q = 0; q < 2; q++) {
+        a.zk[0][q] = c->A0r[2][q];
+        a.zk[1][q] = c->A[2][q];
+        a.zk[2][q] = c->mA[2][q];
+    }
+    a.wz = c->b.wgt + c->Nx + c->Ny;
+    a.wy = c->b.wgt + c->Nx;
+    a.zrows = sweep_z_rows(c);
+}
 
 long sweep_z_tiles(const dbec_ctx *c) {
     const bool f32 = c->prec == DBEC_MIXED;

@@ -45,6 +45,10 @@ hipError_t launch_set_scalar(hipStream_t s, double *p, double v);
 hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves &v, bool cols);
 // number of y tiles of the z sweep (rows of the column partials)
 long sweep_z_tiles(const dbec_ctx *c);
+int sweep_z_rows(const dbec_ctx *c);
+// The z sweep's tables, constants, weights and rows per workgroup for the
+// z sweep fused with the forward z pass.
+void sweep_z_args(const dbec_ctx *c, FftArgs &a);
 // norm of each solve from its column partials: scal[NORM], scal[SCALE]
 hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const Solves &v);
 TrapConst trap_const(const dbec_ctx *c);

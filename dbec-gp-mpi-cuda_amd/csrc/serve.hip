@@ -393,8 +393,11 @@ struct dbec_server {
         v.use_scale = imag ? 1 : 0;
         v.shift = (imag && t->prec == DBEC_MIXED) ? 1 : 0;
         v.fold = 0;
-        Solves vf = v;  // steps after the first fold the previous norm in
-        vf.fold = imag ? 1 : 0;
+        // steps after the first fold the previous norm in; with the z sweep
+        // fused into the next forward z pass, the y pass forms that norm
+        const bool fuse = core_fuse_z(t);
+        Solves vf = v;
+        vf.fold = imag ? (fuse ? 2 : 1) : 0;
         static const bool use_graphs = [] {
             const char *e = getenv("DBEC_SERVE_GRAPHS");
             return !e || atoi(e) != 0;
@@ -406,7 +409,8 @@ struct dbec_server {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
                 int rc = DBEC_OK;
-                for (long k = 0; k < K && e == hipSuccess && rc == DBEC_OK; k++) rc = core_step(t, k ? vf : v);
+                for (long k = 0; k < K && e == hipSuccess && rc == DBEC_OK; k++)
+                    rc = core_step(t, k ? vf : v, fuse && k > 0, fuse && k + 1 < K);
                 if (imag && rc == DBEC_OK) rc = core_norm(t, v);
                 hipError_t e2 = hipStreamEndCapture(st, &g);
                 if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
@@ -419,7 +423,7 @@ struct dbec_server {
             if (e != hipSuccess) return fail(std::string("chunk graph launch: ") + hipGetErrorString(e));
         } else {
             for (long k = 0; k < K; k++) {
-                int rc = core_step(t, k ? vf : v);
+                int rc = core_step(t, k ? vf : v, fuse && k > 0, fuse && k + 1 < K);
                 if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
             }
             if (imag) {

@@ -57,7 +57,7 @@ def short_name(full):
         return "sweep_x" if targs[1] == "0" else "sweep_y"
     return {
         "k_zfwd": "zfwd", "k_xconv": "xconv", "k_zinv_nu": "zinv_nu", "k_zinv": "zinv",
-        "k_sweep_z": "sweep_z", "k_norm_cols": "norm", "k_scale": "scale",
+        "k_sweep_z": "sweep_z", "k_sweepz_zfwd": "sweep_z_zfwd", "k_norm_cols": "norm", "k_scale": "scale",
     }.get(k, k[2:])
 
 
