@@ -11,6 +11,8 @@
 //          multiplication by potdd(|kx|, |ky|, kz), inverse FFT, one pass.
 //   zinv : rebuilds two Hermitian full spectra per complex line, inverse
 //          transform, real and imaginary parts are the two real rows of phi.
+//   zinv_nu: zinv fused with calcnu (the composite step).
+//   sweepz_zfwd: the z sweep of a step fused with zfwd of the next step.
 //
 // Included by fft_inst.hip with DBEC_FFT_N set; every kernel is a template on
 // the length N, the lines per workgroup L and the real type R of spec.
