@@ -223,6 +223,69 @@ __device__ __forceinline__ void stockham(cpx<R> *buf, int LS, int l, int t, cons
     if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, NS * RADIX, REST...>(buf, LS, l, t, tw);
 }
 
+// The same transform with the line in registers at both ends: on entry and on
+// return v[k] holds element t + k T of line l (N / T values per thread).  The
+// first pass (T = N / RADIX: one butterfly per thread, no twiddles) takes its
+// inputs from v, and when its butterflies divide evenly among the T threads
+// the last pass (NS = N / RADIX) leaves its outputs in v: butterfly j = t + q T
+// yields elements j + r NS = t + (q + r Q) T.  The arithmetic is that of
+// stockham(); only LDS round trips and barriers go.  buf must be free on entry
+// (the caller's barrier); on return other threads may still be reading it.
+template <int N, int RADIX, int T, int DIR, typename R>
+__device__ __forceinline__ void stockham_first_reg(cpx<R> *buf, int LS, int l, int t, cpx<R> *v) {
+    static_assert(N / RADIX == T, "first pass: one butterfly per thread");
+    dft<RADIX, DIR>(v);
+#pragma unroll
+    for (int r = 0; r < RADIX; ++r) buf[(t * RADIX + r) * LS + l] = v[r];
+    __syncthreads();
+}
+
+template <int N, int RADIX, int NS, int T, int DIR, typename R>
+__device__ __forceinline__ void stockham_last_reg(const cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw,
+                                                  cpx<R> *out) {
+    constexpr int M = N / RADIX, Q = M / T;
+    static_assert(NS == M && M % T == 0, "last pass: whole butterflies per thread");
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + q * T;
+        cpx<R> v[RADIX];
+#pragma unroll
+        for (int r = 0; r < RADIX; ++r) v[r] = buf[(j + r * M) * LS + l];
+#pragma unroll
+        for (int r = 1; r < RADIX; ++r) {
+            const cpx<R> w = tw[r * j];
+            v[r] = (DIR < 0) ? cmul(v[r], w) : cmulc(v[r], w);
+        }
+        dft<RADIX, DIR>(v);
+#pragma unroll
+        for (int r = 0; r < RADIX; ++r) out[q + r * Q] = v[r];
+    }
+}
+
+template <int N, int T, int DIR, int NS, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_reg_tail(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+    if constexpr (sizeof...(REST) > 0) {
+        stockham_pass<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw);
+        stockham_reg_tail<N, T, DIR, NS * RADIX, REST...>(buf, LS, l, t, tw, v);
+    } else if constexpr ((N / RADIX) % T == 0) {
+        stockham_last_reg<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw, v);
+    } else {
+        stockham_pass<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw);
+#pragma unroll
+        for (int k = 0; k < N / T; ++k) v[k] = buf[(t + k * T) * LS + l];
+    }
+}
+
+template <int N, int T, int DIR, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_reg(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+    if constexpr (sizeof...(REST) == 0) {
+        dft<RADIX, DIR>(v);  // one pass, T = 1: the whole line in one thread
+    } else {
+        stockham_first_reg<N, RADIX, T, DIR>(buf, LS, l, t, v);
+        stockham_reg_tail<N, T, DIR, RADIX, REST...>(buf, LS, l, t, tw, v);
+    }
+}
+
 // Radix plan per supported length.  T = threads per line = N / (first radix).
 // First radices of 4 or 8 give T >= 12 threads per line: fewer values per
 // thread (fewer registers in the fused passes, whose psi tile is 2N / T
@@ -239,6 +302,11 @@ struct FftPlan;
         static __device__ __forceinline__ void run(cpx<R> *buf, int LS, int l, int t,        \
                                                    const cpx<R> *tw) {                       \
             stockham<NN, TT, DIR, 1, __VA_ARGS__>(buf, LS, l, t, tw);                        \
+        }                                                                                    \
+        template <int DIR, typename R>                                                       \
+        static __device__ __forceinline__ void run_reg(cpx<R> *buf, int LS, int l, int t,    \
+                                                       const cpx<R> *tw, cpx<R> *v) {        \
+            stockham_reg<NN, TT, DIR, __VA_ARGS__>(buf, LS, l, t, tw, v);                    \
         }                                                                                    \
     };
 

@@ -362,8 +362,9 @@ template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int T = FftPlan<N>::T, K = N / T;  // x values per thread
+    static_assert(L * T == WG, "xconv tile");
     constexpr int NXH = N / 2 + 1;
-    constexpr int QP = (NXH * L + WG - 1) / WG;  // potdd values per thread
     __shared__ cpx<R> buf[N * LS];
     __shared__ cpx<R> tw[N];
     const SolveAt sr = solve_at(v);
@@ -376,59 +377,35 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     const int y0 = (blockIdx.x % ytiles) * L;
     const int nl = min(L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
-    // potdd(|kx|, |ky|, kz) for kx <= N/2 (table rows [kz][|ky|] contiguous
-    // in kx, lanes along kx), fetched into registers up front; each value
-    // multiplies both kx and N - kx
-    const int Ny2 = Ny / 2, Nyh = Ny2 + 1;
-    const R *pk = potddq + (long)kz * NXH * Nyh;
-    R pw[QP];
-#pragma unroll
-    for (int q = 0; q < QP; ++q) {
-        const int i = tid + q * WG;
-        const int kx = i % NXH, ll = i / NXH;
-        const int y = y0 + ll;
-        const int ky = y <= Ny2 ? y : Ny - y;
-        pw[q] = (i < NXH * L && ll < nl) ? pk[ky * NXH + kx] : R(0);
-    }
-    cpx<R> *g = spec + (long)kz * N * Ny + y0;
-    constexpr int K = L * N / WG;
-    static_assert((L * N) % WG == 0, "xconv tile");
+    // Thread (l, t) holds x = t + k T of y line l, lanes along y (coalesced
+    // rows of L values): the layout in which the transforms take and leave
+    // their line in registers, so the potdd product is applied in registers.
+    const int l = tid % L, t = tid / L;
+    const int y = y0 + (l < nl ? l : 0);
+    cpx<R> *g = spec + (long)kz * N * Ny + y;
     cpx<R> r[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int i = tid + k * WG;
-        const int ll = i % L, e = i / L;
-        r[k] = g[(long)e * Ny + (ll < nl ? ll : 0)];
-    }
+    for (int k = 0; k < K; ++k) r[k] = g[(long)(t + k * T) * Ny];
+    // potdd(|kx|, |ky|, kz): the table rows [kz][|ky|] are contiguous in |kx|
+    const int Ny2 = Ny / 2, Nyh = Ny2 + 1;
+    const R *pk = potddq + ((long)kz * Nyh + (y <= Ny2 ? y : Ny - y)) * NXH;
+    R pw[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int i = tid + k * WG;
-        buf[(i / L) * LS + i % L] = r[k];
+        const int e = t + k * T;
+        pw[k] = pk[e <= N / 2 ? e : N - e];
     }
-    __syncthreads();
-    const int l = tid % L, t = tid / L;
-    FftPlan<N>::template run<-1>(buf, LS, l, t, tw);
-#pragma unroll
-    for (int q = 0; q < QP; ++q) {
-        const int i = tid + q * WG;
-        if (i < NXH * L) {
-            const int kx = i % NXH, ll = i / NXH;
-            const R w = pw[q] * s2;
-            cpx<R> v = buf[kx * LS + ll];
-            buf[kx * LS + ll] = {v.x * w, v.y * w};
-            if (kx > 0 && kx < N / 2) {
-                v = buf[(N - kx) * LS + ll];
-                buf[(N - kx) * LS + ll] = {v.x * w, v.y * w};
-            }
-        }
-    }
-    __syncthreads();
-    FftPlan<N>::template run<+1>(buf, LS, l, t, tw);
+    FftPlan<N>::template run_reg<-1>(buf, LS, l, t, tw, r);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int i = tid + k * WG;
-        const int ll = i % L, e = i / L;
-        if (ll < nl) g[(long)e * Ny + ll] = buf[e * LS + ll];
+        const R w = pw[k] * s2;
+        r[k] = {r[k].x * w, r[k].y * w};
+    }
+    __syncthreads();  // the forward transform's last LDS reads are done
+    FftPlan<N>::template run_reg<+1>(buf, LS, l, t, tw, r);
+    if (l < nl) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) g[(long)(t + k * T) * Ny] = r[k];
     }
 }
 
