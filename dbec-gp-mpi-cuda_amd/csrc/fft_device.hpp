@@ -286,6 +286,13 @@ __device__ __forceinline__ void stockham_reg(cpx<R> *buf, int LS, int l, int t, 
     }
 }
 
+// Registers in (v as for stockham_reg), result in LDS as stockham() leaves it.
+template <int N, int T, int DIR, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_from_reg(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+    stockham_first_reg<N, RADIX, T, DIR>(buf, LS, l, t, v);
+    if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, RADIX, REST...>(buf, LS, l, t, tw);
+}
+
 // Radix plan per supported length.  T = threads per line = N / (first radix).
 // First radices of 4 or 8 give T >= 12 threads per line: fewer values per
 // thread (fewer registers in the fused passes, whose psi tile is 2N / T
@@ -307,6 +314,11 @@ struct FftPlan;
         static __device__ __forceinline__ void run_reg(cpx<R> *buf, int LS, int l, int t,    \
                                                        const cpx<R> *tw, cpx<R> *v) {        \
             stockham_reg<NN, TT, DIR, __VA_ARGS__>(buf, LS, l, t, tw, v);                    \
+        }                                                                                    \
+        template <int DIR, typename R>                                                       \
+        static __device__ __forceinline__ void run_from_reg(cpx<R> *buf, int LS, int l, int t, \
+                                                            const cpx<R> *tw, cpx<R> *v) {   \
+            stockham_from_reg<NN, TT, DIR, __VA_ARGS__>(buf, LS, l, t, tw, v);               \
         }                                                                                    \
     };
 

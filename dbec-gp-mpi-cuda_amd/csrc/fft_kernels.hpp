@@ -451,6 +451,37 @@ __device__ __forceinline__ void load_hermitian_pairs(cpx<R> *buf, const cpx<R> *
     }
 }
 
+// The same pairs straight into the registers of the first transform pass:
+// v[r] = Z[t + r T] of line p = l (T threads per line), each bin above N/2
+// formed from the spectrum rows of its mirror N - k.
+template <int N, typename R, typename SpecIdx>
+__device__ __forceinline__ void load_hermitian_reg(cpx<R> *v, const cpx<R> *__restrict__ spec, int l, int t,
+                                                   const SpecIdx &idx) {
+    constexpr int T = FftPlan<N>::T, K = N / T;
+    cpx<R> ra[K], rb[K];
+    int m[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const int k = t + r * T, kk = k <= N / 2 ? k : N - k;
+        long ia = 0, ib = 0;
+        m[r] = idx(l, kk, ia, ib);
+        ra[r] = spec[(m[r] & 1) ? ia : 0];
+        rb[r] = spec[(m[r] & 2) ? ib : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const int k = t + r * T, kk = k <= N / 2 ? k : N - k;
+        cpx<R> a = ra[r], b = rb[r];
+        if (!(m[r] & 1)) a = {R(0), R(0)};
+        if (!(m[r] & 2)) b = {R(0), R(0)};
+        if (kk == 0 || kk == N / 2) {
+            a.y = R(0);
+            b.y = R(0);
+        }
+        v[r] = k <= N / 2 ? cpx<R>{a.x - b.y, a.y + b.x} : cpx<R>{a.x + b.y, b.x - a.y};
+    }
+}
+
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
@@ -501,8 +532,10 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 // psi of 8 bytes or less per element is fetched into registers before the
 // transform, so its latency overlaps the spectrum load; complex FP64 psi is
 // fetched after it in batches of 8.  The per-row and per-z parts of the trap
-// potential (trap_xy, trap_z) are staged in LDS.
-template <int N, typename R, typename P, int LM>
+// potential (trap_xy, trap_z) are staged in LDS.  With ZREG the spectrum
+// pairs go straight into the registers of the first transform pass
+// (load_hermitian_reg) instead of through LDS; the results are the same.
+template <int N, typename R, typename P, int LM, bool ZREG>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, int nfaceb,
           const cpx<R> *__restrict__ twg) {
@@ -547,13 +580,20 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
                 pv[q] = dst[(long)min(i / N, nrows - 1) * N + i % N];
             }
         }
-        load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+        const auto idx = [&](int p, int k, long &ia, long &ib) {
             ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
             ib = ia + 1;
             return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
-        });
-        __syncthreads();
-        FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+        };
+        if constexpr (ZREG) {
+            cpx<R> zv[N / FftPlan<N>::T];
+            load_hermitian_reg<N>(zv, spec, tid % L, tid / L, idx);
+            FftPlan<N>::template run_from_reg<+1>(buf, LS, tid % L, tid / L, tw, zv);
+        } else {
+            load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, idx);
+            __syncthreads();
+            FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+        }
 #pragma unroll
         for (int q0 = 0; q0 < Q; q0 += C) {
             if constexpr (!PREFETCH) {
@@ -585,7 +625,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         const int f = f0 + min(tid, nl - 1);
         sqr[tid] = trap_xy(tc, sq[f < Ny ? Nx - 1 : f - Ny], sq[Nx + (f < Ny ? f : Ny - 1)]);
     }
-    load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+    const auto fidx = [&](int p, int k, long &ia, long &ib) {
         if (p >= nl) return 0;
         const int f = f0 + p;
         int xo, yo, xs, ys;
@@ -594,9 +634,16 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         ia = ((long)k * Nx + xo) * Ny + yo;
         ib = ((long)k * Nx + xs) * Ny + ys;
         return 3;
-    });
-    __syncthreads();
-    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    };
+    if constexpr (ZREG) {
+        cpx<R> zv[N / FftPlan<N>::T];
+        load_hermitian_reg<N>(zv, spec, tid % L, tid / L, fidx);
+        FftPlan<N>::template run_from_reg<+1>(buf, LS, tid % L, tid / L, tw, zv);
+    } else {
+        load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, fidx);
+        __syncthreads();
+        FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    }
     constexpr int QF = L * N / WG;
     constexpr int CF = PREFETCH ? QF : (QF < 8 ? QF : 8);
     static_assert((L * N) % WG == 0 && QF % CF == 0, "zinv_nu face tile");
@@ -813,7 +860,11 @@ struct FftLaunchImpl {
         const int nmain = (int)((a.Nx - 1) * ytiles);
         const int nfaceb = (int)((a.Ny + a.Nx - 1 + Tl::L - 1) / Tl::L);
         dim3 grid((unsigned)(nmain + nfaceb), (unsigned)a.v.nb), block(Tl::WG);
-        k_zinv_nu<N, R, P, LM><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny, nfaceb, tw);
+        // FP32 spectra load straight into the first pass's registers (faster at
+        // every measured size); FP64 ones through LDS (the extra registers
+        // cost more occupancy than the LDS round trip saves)
+        constexpr bool ZREG = std::is_same<R, float>::value;
+        k_zinv_nu<N, R, P, LM, ZREG><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny, nfaceb, tw);
         return hipGetLastError();
     }
     template <int LM>
