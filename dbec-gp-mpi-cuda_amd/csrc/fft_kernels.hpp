@@ -165,7 +165,7 @@ struct FusedZ {
     // (they are summed before the buffer is filled), else with the sweep's
     // exchange.
     template <typename S, typename R, int L>
-    static constexpr size_t tile_b() { return ((size_t)(N + 1) * L * sizeof(S) + 15) & ~size_t(15); }
+    static constexpr size_t tile_b() { return ((size_t)(N + Pk<S>::n) * L * sizeof(S) + 15) & ~size_t(15); }
     template <typename S, typename R, int L>
     static constexpr size_t x_b() {
         constexpr size_t tile = tile_b<S, R, L>(), buf = (size_t)N * (L / 2 + 1) * sizeof(cpx<R>);
@@ -197,7 +197,10 @@ __global__ void __launch_bounds__(L *(N / 16))
 k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, SweepConst<typename Arith<S>::T> k,
               const double *__restrict__ wz, const double *__restrict__ wy, const cpx<R> *__restrict__ twg) {
     using A = typename Arith<S>::T;
-    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, LS = LF + 1, RS = N + 1;
+    // psi tile rows of N + VE elements: 16-byte aligned, and consecutive rows
+    // 4 banks apart, so the packet accesses below are conflict-free
+    constexpr int VE = Pk<S>::n, KV = 16 / VE;
+    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, LS = LF + 1, RS = N + VE;
     static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
     using F = FusedZ<N>;
     constexpr size_t SH_B = F::template sh_b<S, R, L>(), TILE_B = F::template tile_b<S, R, L>();
@@ -219,7 +222,6 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     const int nl = min(L, Ny - y0);
     const long base = ((long)xrow * Ny + y0) * N;
     const A zero = m_zero(k.A);
-    constexpr int VE = Pk<S>::n, KV = M / VE;
     const int npk = nl * N / VE;
     const CoefPre<A> cp = coef_pre(tab, N, tid);
     {
@@ -236,8 +238,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
             const int iv = tid + q * WG;
             if (iv < npk) {
                 const int e = (iv * VE) % N, row = (iv * VE) / N;
-#pragma unroll
-                for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
+                st_pk(tile + row * RS + e, r[q]);
             }
         }
     }
@@ -248,14 +249,23 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     {
         const S *row = tile + l * RS;
 #pragma unroll
-        for (int j = 0; j < M; ++j) vv[j] = act ? to_a(row[e0 + j]) : zero;
+        for (int m = 0; m < KV; ++m) {
+            const typename Pk<S>::type pk = ld_pk(row + e0 + m * VE);
+#pragma unroll
+            for (int j = 0; j < VE; ++j) vv[m * VE + j] = act ? to_a(pk.e[j]) : zero;
+        }
         const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
         const A hr = (act && e0 + M < N) ? to_a(row[e0 + M]) : zero;
         const double acc = seg_solve<A, M, COLS, true>(vv, hl, hr, e0, N, s, l, L, P, sh, k, wz);
         if (act) {
             S *orow = tile + l * RS;
 #pragma unroll
-            for (int j = 0; j < M; ++j) orow[e0 + j] = from_a<S>(vv[j]);
+            for (int m = 0; m < KV; ++m) {
+                typename Pk<S>::type o;
+#pragma unroll
+                for (int j = 0; j < VE; ++j) o.e[j] = from_a<S>(vv[m * VE + j]);
+                st_pk(orow + e0 + m * VE, o);
+            }
         }
         if constexpr (COLS) {
             sh.exn[s * L + l] = acc;
@@ -279,10 +289,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
         const int iv = tid + q * WG;
         if (iv < npk) {
             const int e = (iv * VE) % N, row = (iv * VE) / N;
-            typename Pk<S>::type o;
-#pragma unroll
-            for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
-            st_pk(psi + base + (long)iv * VE, o);
+            st_pk(psi + base + (long)iv * VE, ld_pk(tile + row * RS + e));
         }
     }
     __syncthreads();  // the tile is stored; its LDS becomes the transform buffer

@@ -191,8 +191,9 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
 
 // z sweep: lines are contiguous rows.  A workgroup covers L rows of one x
 // plane.  Staged (DIRECT false): the rows go through an LDS tile in the
-// storage type (row-major, one element of padding: coalesced row loads and
-// stores, conflict-free segment reads), and each thread moves its segment
+// storage type (row-major, rows padded by one 16-byte packet: coalesced row
+// loads and stores, and 16-byte LDS accesses without bank conflicts both for
+// the row copies and for the segments), and each thread moves its segment
 // between the tile and its registers.  Direct (DIRECT true): each thread
 // loads its segment, M contiguous and 16-byte aligned values of its row,
 // straight into registers as packets and stores it back the same way; the
@@ -210,7 +211,12 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     S *__restrict__ psi = reinterpret_cast<S *>(sr.psi);
     double *__restrict__ colpart = sr.rows;
     const int n = Nz;
-    const int RS = n + 1;  // row stride of the tile
+    constexpr int VE = Pk<S>::n, KV = M / VE;
+    static_assert(M % VE == 0, "z sweep packets");
+    // row stride of the tile: n + VE elements keeps every row 16-byte aligned
+    // and consecutive rows 4 banks apart, so the packet accesses are
+    // conflict-free
+    const int RS = n + VE;
     const int P = n / M;
     const int nt = L * P;
     SweepShared<A> sh(smem_raw, n, nt, L);
@@ -222,8 +228,6 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     const int nl = min(L, Ny - y0);
     const long base = ((long)xrow * Ny + y0) * Nz;
     const A zero = m_zero(k.A);
-    constexpr int VE = Pk<S>::n, KV = M / VE;
-    static_assert(M % VE == 0, "z sweep packets");
     const int npk = nl * n / VE;
     const int e0 = s * M;
     const bool act = l < nl;
@@ -259,14 +263,17 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
             const int iv = tid + q * nt;
             if (iv < npk) {
                 const int e = (iv * VE) % n, row = (iv * VE) / n;
-#pragma unroll
-                for (int j = 0; j < VE; ++j) tile[row * RS + e + j] = r[q].e[j];
+                st_pk(tile + row * RS + e, r[q]);
             }
         }
         __syncthreads();
         const S *row = tile + l * RS;
 #pragma unroll
-        for (int j = 0; j < M; ++j) v[j] = act ? to_a(row[e0 + j]) : zero;
+        for (int q = 0; q < KV; ++q) {
+            const typename Pk<S>::type pk = ld_pk(row + e0 + q * VE);
+#pragma unroll
+            for (int j = 0; j < VE; ++j) v[q * VE + j] = act ? to_a(pk.e[j]) : zero;
+        }
         hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
         hr = (act && e0 + M < n) ? to_a(row[e0 + M]) : zero;
     }
@@ -286,7 +293,12 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     } else if (act) {
         S *orow = tile + l * RS;
 #pragma unroll
-        for (int j = 0; j < M; ++j) orow[e0 + j] = from_a<S>(v[j]);
+        for (int q = 0; q < KV; ++q) {
+            typename Pk<S>::type o;
+#pragma unroll
+            for (int j = 0; j < VE; ++j) o.e[j] = from_a<S>(v[q * VE + j]);
+            st_pk(orow + e0 + q * VE, o);
+        }
     }
     if constexpr (COLS) {
         sh.exn[s * L + l] = acc;
@@ -310,10 +322,7 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
             const int iv = tid + q * nt;
             if (iv < npk) {
                 const int e = (iv * VE) % n, row = (iv * VE) / n;
-                typename Pk<S>::type o;
-#pragma unroll
-                for (int j = 0; j < VE; ++j) o.e[j] = tile[row * RS + e + j];
-                st_pk(psi + base + (long)iv * VE, o);
+                st_pk(psi + base + (long)iv * VE, ld_pk(tile + row * RS + e));
             }
         }
     }
@@ -350,7 +359,7 @@ static int z_rows(long n) {
     const int M = z_seg_len(n);
     const int P = (int)(n / M);
     int L = 64;
-    while (L > 4 && ((long)L * P > max_threads || (long)(n + 1) * L * (long)sizeof(S) > tile_max)) L /= 2;
+    while (L > 4 && ((long)L * P > max_threads || (long)(n + Pk<S>::n) * L * (long)sizeof(S) > tile_max)) L /= 2;
     return L;
 }
 
@@ -466,7 +475,7 @@ static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const 
     const long nblocks = (long)((Ny + L - 1) / L) * Nx;
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
     size_t lds = (SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15);
-    if (!DIRECT) lds += (size_t)(Nz + 1) * L * sizeof(S);
+    if (!DIRECT) lds += (size_t)(Nz + Pk<S>::n) * L * sizeof(S);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_z<S, L, 16, COLS, DIRECT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
