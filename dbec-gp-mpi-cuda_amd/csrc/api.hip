@@ -598,7 +598,28 @@ int autotune(dbec_ctx *c) {
     CHK(choose(&t.lm[0], {1, 2}, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }));
     CHK(choose(&t.lm[1], {1, 2}, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }));
     CHK(choose(&t.lm[2], {1, 2}, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }));
-    CHK(choose(&t.lm[3], {1, 2}, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }));
+    if (c->mode == DBEC_IMAG && c->prec != DBEC_MIXED) {
+        // the fused inverse pass has two buffer layouts here: its tile
+        // multiplier and layout are chosen together
+        double best = 1e30;
+        int blm = 1, bsw = 0;
+        for (int sw : {0, 1})
+            for (int lm : {1, 2}) {
+                t.zinv_swz = sw;
+                t.lm[3] = lm;
+                double ms;
+                CHK(time_launches(c, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }, &ms));
+                if (ms < best) {
+                    best = ms;
+                    blm = lm;
+                    bsw = sw;
+                }
+            }
+        t.lm[3] = blm;
+        t.zinv_swz = bsw;
+    } else {
+        CHK(choose(&t.lm[3], {1, 2}, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }));
+    }
     for (int axis = 0; axis < 2; axis++) {
         auto sweep = [&] { return launch_sweep(c->stream, c, axis, v, false); };
         CHK(choose(&t.xy_peel[axis], {0, 1}, sweep));
@@ -608,10 +629,10 @@ int autotune(dbec_ctx *c) {
     // the candidates ran on zero data; psi and the spectrum stay zero
     if (const char *lg = getenv("DBEC_AUTOTUNE_LOG"); lg && std::strcmp(lg, "0") != 0)
         std::fprintf(stderr, "[dbec autotune] %ldx%ldx%ld %s %s: lm zfwd %d ypass %d xconv %d fused %d; "
-                             "sweep x rowb %d peel %d, y rowb %d peel %d, z direct %d\n",
+                             "sweep x rowb %d peel %d, y rowb %d peel %d, z direct %d; fused layout %d\n",
                      c->Nx, c->Ny, c->Nz, c->mode == DBEC_IMAG ? "imag" : "real",
                      c->prec == DBEC_MIXED ? "mixed" : "fp64", t.lm[0], t.lm[1], t.lm[2], t.lm[3], t.xy_rowb[0],
-                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1], t.z_direct);
+                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1], t.z_direct, t.zinv_swz);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = t;
     return DBEC_OK;

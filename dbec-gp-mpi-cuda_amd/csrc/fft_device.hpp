@@ -1,9 +1,9 @@
 // fft_device.hpp - LDS-resident Stockham FFT building blocks for gfx950.
 //
-// A workgroup holds L lines of length N in LDS, element e of line l at
-// buf[e * LS + l] (LS >= L, line index fastest).  Every line is transformed by
-// T threads; thread (l, t) has l = tid % L, t = tid / L, so the lanes of a
-// wavefront touch consecutive LDS slots in every pass (no bank conflicts).
+// A workgroup holds L lines of length N in LDS, element e of line l in slot
+// lds_at(e, l): e-major, line index fastest, padded or XOR-swizzled against
+// bank conflicts.  Every line is transformed by T threads; thread (l, t) has
+// l = tid % L, t = tid / L.
 // Pass p of radix R (span NS = product of the earlier radices) computes
 // butterfly j from inputs j + r*N/R and writes outputs to
 // (j / NS) * NS * R + j % NS + r * NS (Stockham autosort: natural order in and
@@ -181,9 +181,40 @@ __device__ __forceinline__ void dft(cpx<R> *v) {
     else static_assert(RADIX == 2, "unsupported radix");
 }
 
+constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
+
+// LDS slot of element e of line l in a buffer of L transform lines for the
+// plan of length N with T threads per line.  Padded (SWZ false):
+// e (L + 1) + l, N (L + 1) slots.  Swizzled (SWZ true, L a power of two):
+//   (e ^ ((e >> log2 R1) & 3)) * L + (l ^ ((e >> S1) & (L - 1))),
+// R1 = N / T the first radix, S1 = log2 of the lines' rows per 256 bytes, a
+// permutation of [0, N L).  Its row swizzle spreads the first pass's outputs
+// (rows R1 t + r of four consecutive t) over four bank groups, its column
+// swizzle spreads a run of consecutive e of one line (the coalesced row
+// copies) over all banks, and the other passes touch four consecutive rows of
+// every line at once; on 64 banks of 4 bytes, serving 256 bytes per cycle,
+// that removes the bank conflicts the padded layout has in the passes.  It
+// costs address registers: measured, it pays only where the conflicts
+// dominate (FP64 zinv_nu with real psi).
+template <int L, bool SWZ>
+constexpr int lds_slots_per_row() { return SWZ ? L : L + 1; }
+
+template <int N, int T, int L, typename R, bool SWZ>
+__device__ __forceinline__ int lds_at(int e, int l) {
+    if constexpr (!SWZ) {
+        return e * (L + 1) + l;
+    } else {
+        static_assert((L & (L - 1)) == 0, "lines per buffer: a power of two");
+        constexpr int SH = ilog2(N / T);
+        constexpr int RB = 256 / (L * (int)sizeof(cpx<R>));
+        constexpr int S1 = RB > 1 ? ilog2(RB) : 0;
+        return (e ^ ((e >> SH) & 3)) * L + (l ^ ((e >> S1) & (L - 1)));
+    }
+}
+
 // One Stockham pass over the L lines held in LDS.
-template <int N, int RADIX, int NS, int T, int DIR, typename R>
-__device__ __forceinline__ void stockham_pass(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw) {
+template <int N, int RADIX, int NS, int T, int DIR, int L, bool SWZ, typename R>
+__device__ __forceinline__ void stockham_pass(cpx<R> *buf, int l, int t, const cpx<R> *tw) {
     constexpr int M = N / RADIX;
     constexpr int Q = (M + T - 1) / T;
     cpx<R> v[Q][RADIX];
@@ -192,7 +223,7 @@ __device__ __forceinline__ void stockham_pass(cpx<R> *buf, int LS, int l, int t,
         const int j = t + q * T;
         if (M % T == 0 || j < M) {
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) v[q][r] = buf[(j + r * M) * LS + l];
+            for (int r = 0; r < RADIX; ++r) v[q][r] = buf[lds_at<N, T, L, R, SWZ>(j + r * M, l)];
         }
     }
     __syncthreads();
@@ -211,16 +242,16 @@ __device__ __forceinline__ void stockham_pass(cpx<R> *buf, int LS, int l, int t,
             dft<RADIX, DIR>(v[q]);
             const int base = (j / NS) * NS * RADIX + k;
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) buf[(base + r * NS) * LS + l] = v[q][r];
+            for (int r = 0; r < RADIX; ++r) buf[lds_at<N, T, L, R, SWZ>(base + r * NS, l)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-template <int N, int T, int DIR, int NS, int RADIX, int... REST, typename R>
-__device__ __forceinline__ void stockham(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw) {
-    stockham_pass<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw);
-    if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, NS * RADIX, REST...>(buf, LS, l, t, tw);
+template <int N, int T, int DIR, int L, bool SWZ, int NS, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham(cpx<R> *buf, int l, int t, const cpx<R> *tw) {
+    stockham_pass<N, RADIX, NS, T, DIR, L, SWZ>(buf, l, t, tw);
+    if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, L, SWZ, NS * RADIX, REST...>(buf, l, t, tw);
 }
 
 // The same transform with the line in registers at both ends: on entry and on
@@ -231,18 +262,17 @@ __device__ __forceinline__ void stockham(cpx<R> *buf, int LS, int l, int t, cons
 // yields elements j + r NS = t + (q + r Q) T.  The arithmetic is that of
 // stockham(); only LDS round trips and barriers go.  buf must be free on entry
 // (the caller's barrier); on return other threads may still be reading it.
-template <int N, int RADIX, int T, int DIR, typename R>
-__device__ __forceinline__ void stockham_first_reg(cpx<R> *buf, int LS, int l, int t, cpx<R> *v) {
+template <int N, int RADIX, int T, int DIR, int L, bool SWZ, typename R>
+__device__ __forceinline__ void stockham_first_reg(cpx<R> *buf, int l, int t, cpx<R> *v) {
     static_assert(N / RADIX == T, "first pass: one butterfly per thread");
     dft<RADIX, DIR>(v);
 #pragma unroll
-    for (int r = 0; r < RADIX; ++r) buf[(t * RADIX + r) * LS + l] = v[r];
+    for (int r = 0; r < RADIX; ++r) buf[lds_at<N, T, L, R, SWZ>(t * RADIX + r, l)] = v[r];
     __syncthreads();
 }
 
-template <int N, int RADIX, int NS, int T, int DIR, typename R>
-__device__ __forceinline__ void stockham_last_reg(const cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw,
-                                                  cpx<R> *out) {
+template <int N, int RADIX, int NS, int T, int DIR, int L, bool SWZ, typename R>
+__device__ __forceinline__ void stockham_last_reg(const cpx<R> *buf, int l, int t, const cpx<R> *tw, cpx<R> *out) {
     constexpr int M = N / RADIX, Q = M / T;
     static_assert(NS == M && M % T == 0, "last pass: whole butterflies per thread");
 #pragma unroll
@@ -250,7 +280,7 @@ __device__ __forceinline__ void stockham_last_reg(const cpx<R> *buf, int LS, int
         const int j = t + q * T;
         cpx<R> v[RADIX];
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) v[r] = buf[(j + r * M) * LS + l];
+        for (int r = 0; r < RADIX; ++r) v[r] = buf[lds_at<N, T, L, R, SWZ>(j + r * M, l)];
 #pragma unroll
         for (int r = 1; r < RADIX; ++r) {
             const cpx<R> w = tw[r * j];
@@ -262,42 +292,43 @@ __device__ __forceinline__ void stockham_last_reg(const cpx<R> *buf, int LS, int
     }
 }
 
-template <int N, int T, int DIR, int NS, int RADIX, int... REST, typename R>
-__device__ __forceinline__ void stockham_reg_tail(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+template <int N, int T, int DIR, int L, bool SWZ, int NS, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_reg_tail(cpx<R> *buf, int l, int t, const cpx<R> *tw, cpx<R> *v) {
     if constexpr (sizeof...(REST) > 0) {
-        stockham_pass<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw);
-        stockham_reg_tail<N, T, DIR, NS * RADIX, REST...>(buf, LS, l, t, tw, v);
+        stockham_pass<N, RADIX, NS, T, DIR, L, SWZ>(buf, l, t, tw);
+        stockham_reg_tail<N, T, DIR, L, SWZ, NS * RADIX, REST...>(buf, l, t, tw, v);
     } else if constexpr ((N / RADIX) % T == 0) {
-        stockham_last_reg<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw, v);
+        stockham_last_reg<N, RADIX, NS, T, DIR, L, SWZ>(buf, l, t, tw, v);
     } else {
-        stockham_pass<N, RADIX, NS, T, DIR>(buf, LS, l, t, tw);
+        stockham_pass<N, RADIX, NS, T, DIR, L, SWZ>(buf, l, t, tw);
 #pragma unroll
-        for (int k = 0; k < N / T; ++k) v[k] = buf[(t + k * T) * LS + l];
+        for (int k = 0; k < N / T; ++k) v[k] = buf[lds_at<N, T, L, R, SWZ>(t + k * T, l)];
     }
 }
 
-template <int N, int T, int DIR, int RADIX, int... REST, typename R>
-__device__ __forceinline__ void stockham_reg(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+template <int N, int T, int DIR, int L, bool SWZ, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_reg(cpx<R> *buf, int l, int t, const cpx<R> *tw, cpx<R> *v) {
     if constexpr (sizeof...(REST) == 0) {
         dft<RADIX, DIR>(v);  // one pass, T = 1: the whole line in one thread
     } else {
-        stockham_first_reg<N, RADIX, T, DIR>(buf, LS, l, t, v);
-        stockham_reg_tail<N, T, DIR, RADIX, REST...>(buf, LS, l, t, tw, v);
+        stockham_first_reg<N, RADIX, T, DIR, L, SWZ>(buf, l, t, v);
+        stockham_reg_tail<N, T, DIR, L, SWZ, RADIX, REST...>(buf, l, t, tw, v);
     }
 }
 
 // Registers in (v as for stockham_reg), result in LDS as stockham() leaves it.
-template <int N, int T, int DIR, int RADIX, int... REST, typename R>
-__device__ __forceinline__ void stockham_from_reg(cpx<R> *buf, int LS, int l, int t, const cpx<R> *tw, cpx<R> *v) {
-    stockham_first_reg<N, RADIX, T, DIR>(buf, LS, l, t, v);
-    if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, RADIX, REST...>(buf, LS, l, t, tw);
+template <int N, int T, int DIR, int L, bool SWZ, int RADIX, int... REST, typename R>
+__device__ __forceinline__ void stockham_from_reg(cpx<R> *buf, int l, int t, const cpx<R> *tw, cpx<R> *v) {
+    stockham_first_reg<N, RADIX, T, DIR, L, SWZ>(buf, l, t, v);
+    if constexpr (sizeof...(REST) > 0) stockham<N, T, DIR, L, SWZ, RADIX, REST...>(buf, l, t, tw);
 }
 
 // Radix plan per supported length.  T = threads per line = N / (first radix).
 // First radices of 4 or 8 give T >= 12 threads per line: fewer values per
 // thread (fewer registers in the fused passes, whose psi tile is 2N / T
 // values per thread) and workgroups of whole wavefronts; at every length but
-// 384 these measured faster than plans that start with radix 16.
+// 384 these measured faster than plans that start with radix 16.  at<L, R>()
+// is the LDS slot of element e of line l (lds_at).
 template <int N>
 struct FftPlan;
 
@@ -305,20 +336,23 @@ struct FftPlan;
     template <>                                                                              \
     struct FftPlan<NN> {                                                                     \
         static constexpr int T = TT;                                                         \
-        template <int DIR, typename R>                                                       \
-        static __device__ __forceinline__ void run(cpx<R> *buf, int LS, int l, int t,        \
-                                                   const cpx<R> *tw) {                       \
-            stockham<NN, TT, DIR, 1, __VA_ARGS__>(buf, LS, l, t, tw);                        \
+        template <int L, typename R, bool SWZ = false>                                       \
+        static __device__ __forceinline__ int at(int e, int l) {                             \
+            return lds_at<NN, TT, L, R, SWZ>(e, l);                                          \
         }                                                                                    \
-        template <int DIR, typename R>                                                       \
-        static __device__ __forceinline__ void run_reg(cpx<R> *buf, int LS, int l, int t,    \
+        template <int DIR, int L, bool SWZ = false, typename R>                              \
+        static __device__ __forceinline__ void run(cpx<R> *buf, int l, int t, const cpx<R> *tw) { \
+            stockham<NN, TT, DIR, L, SWZ, 1, __VA_ARGS__>(buf, l, t, tw);                    \
+        }                                                                                    \
+        template <int DIR, int L, bool SWZ = false, typename R>                              \
+        static __device__ __forceinline__ void run_reg(cpx<R> *buf, int l, int t,            \
                                                        const cpx<R> *tw, cpx<R> *v) {        \
-            stockham_reg<NN, TT, DIR, __VA_ARGS__>(buf, LS, l, t, tw, v);                    \
+            stockham_reg<NN, TT, DIR, L, SWZ, __VA_ARGS__>(buf, l, t, tw, v);                \
         }                                                                                    \
-        template <int DIR, typename R>                                                       \
-        static __device__ __forceinline__ void run_from_reg(cpx<R> *buf, int LS, int l, int t, \
+        template <int DIR, int L, bool SWZ = false, typename R>                              \
+        static __device__ __forceinline__ void run_from_reg(cpx<R> *buf, int l, int t,      \
                                                             const cpx<R> *tw, cpx<R> *v) {   \
-            stockham_from_reg<NN, TT, DIR, __VA_ARGS__>(buf, LS, l, t, tw, v);               \
+            stockham_from_reg<NN, TT, DIR, L, SWZ, __VA_ARGS__>(buf, l, t, tw, v);           \
         }                                                                                    \
     };
 

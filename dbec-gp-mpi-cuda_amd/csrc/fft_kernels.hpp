@@ -76,8 +76,9 @@ struct PsiTraits<cpx<float>> {
 template <int N, typename R, typename P, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *__restrict__ wx, int tiles) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
-    __shared__ cpx<R> buf[N * LS];
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
+    using Pl = FftPlan<N>;
+    __shared__ cpx<R> buf[N * (L + 1)];
     __shared__ cpx<R> tw[N];
     const SolveAt sr = solve_at(v);
     const P *__restrict__ psi = reinterpret_cast<const P *>(sr.psi);
@@ -91,8 +92,8 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
     stage_twiddles(tw, twg, N, tid, WG);
     // Rows 2p and 2p+1 of the tile become the real and imaginary parts of
     // line p.  A thread takes VE consecutive z of both rows, so its LDS
-    // writes are whole complex slots (stride LS: conflict-free); all of its
-    // loads are issued before the first LDS write.
+    // writes are whole complex slots; all of its loads are issued before the
+    // first LDS write.
     constexpr int VE = sizeof(P) == 4 ? 2 : 1, NVR = N / VE, KV = L * NVR / WG;
     static_assert((L * NVR) % WG == 0, "zfwd tile");
     struct alignas(VE * sizeof(P)) Pv {
@@ -115,11 +116,11 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
         for (int j = 0; j < VE; ++j) {
             const R va = 2 * p < nrows ? (R)PsiTraits<P>::mod2(ra[k].e[j], s) : R(0);
             const R vb = 2 * p + 1 < nrows ? (R)PsiTraits<P>::mod2(rb[k].e[j], s) : R(0);
-            buf[(zv * VE + j) * LS + p] = {va, vb};
+            buf[Pl::template at<L, R>(zv * VE + j, p)] = {va, vb};
         }
     }
     __syncthreads();
-    FftPlan<N>::template run<-1>(buf, LS, tid % L, tid / L, tw);
+    Pl::template run<-1, L>(buf, tid % L, tid / L, tw);
     // split Z = A + iB into the spectra of the two real rows:
     //   A[k] = (Z[k] + conj Z[N-k]) / 2,  B[k] = (Z[k] - conj Z[N-k]) / (2i)
     constexpr int nkz = N / 2 + 1;
@@ -129,8 +130,8 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
         const int row = i % (2 * L), k = i / (2 * L);
         if (k < nkz && row < nrows) {
             const int p = row >> 1;
-            const cpx<R> z = buf[k * LS + p];
-            const cpx<R> w = buf[((N - k) % N) * LS + p];
+            const cpx<R> z = buf[Pl::template at<L, R>(k, p)];
+            const cpx<R> w = buf[Pl::template at<L, R>((N - k) % N, p)];
             cpx<R> o;
             if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
             else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
@@ -200,12 +201,13 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     // psi tile rows of N + VE elements: 16-byte aligned, and consecutive rows
     // 4 banks apart, so the packet accesses below are conflict-free
     constexpr int VE = Pk<S>::n, KV = 16 / VE;
-    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, LS = LF + 1, RS = N + VE;
+    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, RS = N + VE;
+    using Pl = FftPlan<N>;
     static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
     using F = FusedZ<N>;
     constexpr size_t SH_B = F::template sh_b<S, R, L>(), TILE_B = F::template tile_b<S, R, L>();
     constexpr size_t X_B = F::template x_b<S, R, L>();
-    static_assert(TILE_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * LS * sizeof(cpx<R>), "fused z LDS");
+    static_assert(TILE_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * (LF + 1) * sizeof(cpx<R>), "fused z LDS");
     __shared__ __align__(16) char smem[F::template lds<S, R, L>()];
     const SolveAt sr = solve_at(v);
     S *__restrict__ psi = reinterpret_cast<S *>(sr.psi);
@@ -298,11 +300,11 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         const R m2 = act ? (R)PsiTraits<S>::mod2(from_a<S>(vv[j]), 1.0) : R(0);
-        R *slot = reinterpret_cast<R *>(&buf[(e0 + j) * LS + (l >> 1)]);
+        R *slot = reinterpret_cast<R *>(&buf[Pl::template at<LF, R>(e0 + j, l >> 1)]);
         slot[l & 1] = m2;
     }
     __syncthreads();
-    FftPlan<N>::template run<-1>(buf, LS, tid % LF, tid / LF, tw);
+    Pl::template run<-1, LF>(buf, tid % LF, tid / LF, tw);
     // split Z = A + iB into the spectra of the two real rows (as k_zfwd)
     constexpr int nkz = N / 2 + 1;
 #pragma unroll
@@ -311,8 +313,8 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
         const int row = i % L, kz = i / L;
         if (kz < nkz && row < nl) {
             const int p = row >> 1;
-            const cpx<R> z = buf[kz * LS + p];
-            const cpx<R> w = buf[((N - kz) % N) * LS + p];
+            const cpx<R> z = buf[Pl::template at<LF, R>(kz, p)];
+            const cpx<R> w = buf[Pl::template at<LF, R>((N - kz) % N, p)];
             cpx<R> o;
             if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
             else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
@@ -328,8 +330,9 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
 template <int N, int DIR, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg, const double *__restrict__ wx, int tiles, int Nx) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
-    __shared__ cpx<R> buf[N * LS];
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
+    using Pl = FftPlan<N>;
+    __shared__ cpx<R> buf[N * (L + 1)];
     __shared__ cpx<R> tw[N];
     cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(solve_at(v).spec);
     const int tid = threadIdx.x;
@@ -348,14 +351,14 @@ k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg, const double *__r
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = tid + k * WG;
-        buf[(i % N) * LS + i / N] = r[k];
+        buf[Pl::template at<L, R>(i % N, i / N)] = r[k];
     }
     __syncthreads();
-    FftPlan<N>::template run<DIR>(buf, LS, tid % L, tid / L, tw);
+    Pl::template run<DIR, L>(buf, tid % L, tid / L, tw);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = tid + k * WG;
-        if (i < nl * N) g[i] = buf[(i % N) * LS + i / N];
+        if (i < nl * N) g[i] = buf[Pl::template at<L, R>(i % N, i / N)];
     }
     if (DIR < 0 && v.fold == 2 && blockIdx.x == 0) {
         const SolveAt sr = solve_at(v);
@@ -368,11 +371,11 @@ k_ypass(Solves v, long nlines, const cpx<R> *__restrict__ twg, const double *__r
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
     constexpr int T = FftPlan<N>::T, K = N / T;  // x values per thread
     static_assert(L * T == WG, "xconv tile");
     constexpr int NXH = N / 2 + 1;
-    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> buf[N * (L + 1)];
     __shared__ cpx<R> tw[N];
     const SolveAt sr = solve_at(v);
     cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(sr.spec);
@@ -402,14 +405,14 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
         const int e = t + k * T;
         pw[k] = pk[e <= N / 2 ? e : N - e];
     }
-    FftPlan<N>::template run_reg<-1>(buf, LS, l, t, tw, r);
+    FftPlan<N>::template run_reg<-1, L>(buf, l, t, tw, r);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const R w = pw[k] * s2;
         r[k] = {r[k].x * w, r[k].y * w};
     }
     __syncthreads();  // the forward transform's last LDS reads are done
-    FftPlan<N>::template run_reg<+1>(buf, LS, l, t, tw, r);
+    FftPlan<N>::template run_reg<+1, L>(buf, l, t, tw, r);
     if (l < nl) {
 #pragma unroll
         for (int k = 0; k < K; ++k) g[(long)(t + k * T) * Ny] = r[k];
@@ -424,7 +427,7 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
 // the spec indices of A and B and returns which of them exist (bit 0: A,
 // bit 1: B); a missing one reads as zero.  All loads are issued before the
 // first LDS write.
-template <int N, int L, int WG, int LS, typename R, typename SpecIdx>
+template <int N, int L, int WG, bool SWZ, typename R, typename SpecIdx>
 __device__ __forceinline__ void load_hermitian_pairs(cpx<R> *buf, const cpx<R> *__restrict__ spec, int tid,
                                                      const SpecIdx &idx) {
     constexpr int nkz = N / 2 + 1;
@@ -452,8 +455,8 @@ __device__ __forceinline__ void load_hermitian_pairs(cpx<R> *buf, const cpx<R> *
                 a.y = R(0);
                 b.y = R(0);
             }
-            buf[k * LS + p] = {a.x - b.y, a.y + b.x};
-            if (k > 0 && k < N / 2) buf[(N - k) * LS + p] = {a.x + b.y, b.x - a.y};
+            buf[FftPlan<N>::template at<L, R, SWZ>(k, p)] = {a.x - b.y, a.y + b.x};
+            if (k > 0 && k < N / 2) buf[FftPlan<N>::template at<L, R, SWZ>(N - k, p)] = {a.x + b.y, b.x - a.y};
         }
     }
 }
@@ -492,8 +495,9 @@ __device__ __forceinline__ void load_hermitian_reg(cpx<R> *v, const cpx<R> *__re
 template <int N, typename R, int LM>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
-    __shared__ cpx<R> buf[N * LS];
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
+    using Pl = FftPlan<N>;
+    __shared__ cpx<R> buf[N * (L + 1)];
     __shared__ cpx<R> tw[N];
     const int tid = threadIdx.x;
     const int ytiles = (Ny + 2 * L - 1) / (2 * L);
@@ -501,13 +505,13 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
     const int y0 = (blockIdx.x % ytiles) * 2 * L;
     const int nrows = min(2 * L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
-    load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
+    load_hermitian_pairs<N, L, WG, false>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
         ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
         ib = ia + 1;
         return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
     });
     __syncthreads();
-    FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+    Pl::template run<+1, L>(buf, tid % L, tid / L, tw);
     R *dst = phi + ((long)x * Ny + y0) * N;
     constexpr int Q = 2 * L * N / WG;
 #pragma unroll
@@ -515,7 +519,7 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
         const int i = tid + q * WG;
         const int z = i % N, row = i / N;
         if (row < nrows) {
-            const cpx<R> c = buf[z * LS + (row >> 1)];
+            const cpx<R> c = buf[Pl::template at<L, R>(z, row >> 1)];
             dst[(long)row * N + z] = (row & 1) ? c.y : c.x;
         }
     }
@@ -541,17 +545,19 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 // fetched after it in batches of 8.  The per-row and per-z parts of the trap
 // potential (trap_xy, trap_z) are staged in LDS.  With ZREG the spectrum
 // pairs go straight into the registers of the first transform pass
-// (load_hermitian_reg) instead of through LDS; the results are the same.
-template <int N, typename R, typename P, int LM, bool ZREG>
+// (load_hermitian_reg) instead of through LDS; with SWZ the transform buffer
+// is the swizzled one (lds_at).  Neither changes the results.
+template <int N, typename R, typename P, int LM, bool ZREG, bool SWZ>
 __global__ void __launch_bounds__((FftTile<N, LM>::WG))
 k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, int nfaceb,
           const cpx<R> *__restrict__ twg) {
-    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG, LS = L + 1;
+    constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
+    using Pl = FftPlan<N>;
     constexpr int Q = 2 * L * N / WG;  // psi elements per thread (interior rows)
     constexpr bool PREFETCH = sizeof(P) <= 8;
     constexpr int C = PREFETCH ? Q : (Q < 8 ? Q : 8);  // psi batch
     static_assert((2 * L * N) % WG == 0 && Q % C == 0, "zinv_nu tile");
-    __shared__ cpx<R> buf[N * LS];
+    __shared__ cpx<R> buf[N * lds_slots_per_row<L, SWZ>()];
     __shared__ cpx<R> tw[N];
     __shared__ double sqz[N];      // trap_z of each z
     __shared__ double sqr[2 * L];  // trap_xy of the tile's rows (faces: of each face row)
@@ -595,11 +601,11 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         if constexpr (ZREG) {
             cpx<R> zv[N / FftPlan<N>::T];
             load_hermitian_reg<N>(zv, spec, tid % L, tid / L, idx);
-            FftPlan<N>::template run_from_reg<+1>(buf, LS, tid % L, tid / L, tw, zv);
+            Pl::template run_from_reg<+1, L, SWZ>(buf, tid % L, tid / L, tw, zv);
         } else {
-            load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, idx);
+            load_hermitian_pairs<N, L, WG, SWZ>(buf, spec, tid, idx);
             __syncthreads();
-            FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+            Pl::template run<+1, L, SWZ>(buf, tid % L, tid / L, tw);
         }
 #pragma unroll
         for (int q0 = 0; q0 < Q; q0 += C) {
@@ -615,7 +621,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
                 const int i = tid + (q0 + c) * WG;
                 const int z = i % N, row = i / N;
                 if (row < nrows && y0 + row < Ny - 1) {
-                    const cpx<R> cv = buf[(z == N - 1 ? 0 : z) * LS + (row >> 1)];
+                    const cpx<R> cv = buf[Pl::template at<L, R, SWZ>(z == N - 1 ? 0 : z, row >> 1)];
                     const R phi = (row & 1) ? cv.y : cv.x;
                     const double pot = trap_sum(tc, sqr[row], sqz[z]);
                     dst[(long)row * N + z] = nu_apply(pv[PREFETCH ? q0 + c : c], s, phi, pot, nc);
@@ -645,11 +651,11 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     if constexpr (ZREG) {
         cpx<R> zv[N / FftPlan<N>::T];
         load_hermitian_reg<N>(zv, spec, tid % L, tid / L, fidx);
-        FftPlan<N>::template run_from_reg<+1>(buf, LS, tid % L, tid / L, tw, zv);
+        Pl::template run_from_reg<+1, L, SWZ>(buf, tid % L, tid / L, tw, zv);
     } else {
-        load_hermitian_pairs<N, L, WG, LS>(buf, spec, tid, fidx);
+        load_hermitian_pairs<N, L, WG, SWZ>(buf, spec, tid, fidx);
         __syncthreads();
-        FftPlan<N>::template run<+1>(buf, LS, tid % L, tid / L, tw);
+        Pl::template run<+1, L, SWZ>(buf, tid % L, tid / L, tw);
     }
     constexpr int QF = L * N / WG;
     constexpr int CF = PREFETCH ? QF : (QF < 8 ? QF : 8);
@@ -673,7 +679,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
             if (p < nl) {
                 const int f = f0 + p;
                 const bool from_src = (f < Ny) ? (f < Ny - 1) && (z < N - 1) : (z < N - 1);
-                const cpx<R> cv = buf[z * LS + p];
+                const cpx<R> cv = buf[Pl::template at<L, R, SWZ>(z, p)];
                 const R phi = from_src ? cv.y : cv.x;
                 const double pot = trap_sum(tc, sqr[p], sqz[z]);
                 psi[fn[c]] = nu_apply(fv[c], s, phi, pot, nc);
@@ -691,6 +697,14 @@ inline int fft_lm(const char *name, int dflt) {
     if (!e) return dflt;
     int v = atoi(e);
     return (v == 1 || v == 2) ? v : dflt;
+}
+// DBEC_FFT_ZINV_SWZ: transform buffer layout of the FP64 fused inverse pass
+// with real psi (0 padded, 1 swizzled); -1 when unset
+inline int fft_swz_env() {
+    const char *e = getenv("DBEC_FFT_ZINV_SWZ");
+    if (!e) return -1;
+    const int v = atoi(e);
+    return (v == 0 || v == 1) ? v : -1;
 }
 
 template <int N>
@@ -869,9 +883,21 @@ struct FftLaunchImpl {
         dim3 grid((unsigned)(nmain + nfaceb), (unsigned)a.v.nb), block(Tl::WG);
         // FP32 spectra load straight into the first pass's registers (faster at
         // every measured size); FP64 ones through LDS (the extra registers
-        // cost more occupancy than the LDS round trip saves)
+        // cost more occupancy than the LDS round trip saves).  FP64 with real
+        // psi has both transform buffer layouts (faster at 256, slower at 160
+        // and 192): the environment variable, else the tuned choice, else
+        // padded.
         constexpr bool ZREG = std::is_same<R, float>::value;
-        k_zinv_nu<N, R, P, LM, ZREG><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny, nfaceb, tw);
+        if constexpr (std::is_same<R, double>::value && std::is_same<P, double>::value) {
+            static const int env = fft_swz_env();
+            if (env >= 0 ? env == 1 : a.tune.zinv_swz == 1) {
+                k_zinv_nu<N, R, P, LM, ZREG, true><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny,
+                                                                                  nfaceb, tw);
+                return hipGetLastError();
+            }
+        }
+        k_zinv_nu<N, R, P, LM, ZREG, false><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny,
+                                                                           nfaceb, tw);
         return hipGetLastError();
     }
     template <int LM>

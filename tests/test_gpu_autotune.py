@@ -1,5 +1,6 @@
 """The launch choices that the autotuner picks (FFT tile multipliers, x/y
-sweep row width and boundary peeling, z sweep staged or direct) change no result: the same solves run
+sweep row width and boundary peeling, z sweep staged or direct, the LDS
+layout of the FP64 fused inverse pass) change no result: the same solves run
 in child processes with the built-in rules, with every choice forced the
 other way, and with the tuner on, and psi must agree bit for bit."""
 import json
@@ -38,15 +39,16 @@ print(json.dumps(out))
 FORCED = {
     "one": {**{f"DBEC_FFT_LM_{p}{b}": "1" for p in ("ZFWD", "YPASS", "XCONV", "FUSED") for b in (64, 32)},
             "DBEC_SWEEP_XY_ROWB": "128", "DBEC_SWEEP_XY_ROWB32": "128", "DBEC_SWEEP_XY_PEEL": "0",
-            "DBEC_SWEEP_Z_DIRECT": "1"},
+            "DBEC_SWEEP_Z_DIRECT": "1", "DBEC_FFT_ZINV_SWZ": "1"},
     "two": {**{f"DBEC_FFT_LM_{p}{b}": "2" for p in ("ZFWD", "YPASS", "XCONV", "FUSED") for b in (64, 32)},
             "DBEC_SWEEP_XY_ROWB": "256", "DBEC_SWEEP_XY_ROWB32": "256", "DBEC_SWEEP_XY_PEEL": "1",
-            "DBEC_SWEEP_Z_DIRECT": "0"},
+            "DBEC_SWEEP_Z_DIRECT": "0", "DBEC_FFT_ZINV_SWZ": "0"},
 }
 
 
 def run(extra):
-    env = {k: v for k, v in os.environ.items() if not k.startswith(("DBEC_FFT_LM_", "DBEC_SWEEP_", "DBEC_AUTOTUNE"))}
+    env = {k: v for k, v in os.environ.items()
+           if not k.startswith(("DBEC_FFT_LM_", "DBEC_FFT_ZINV_SWZ", "DBEC_SWEEP_", "DBEC_AUTOTUNE"))}
     env.update(extra)
     print(f"[gpu-case] autotune child {sorted(extra.items())[:2]}...", flush=True)
     r = subprocess.run([sys.executable, "-c", SCRIPT, os.path.join(PKG, "python")], env=env, capture_output=True,
