@@ -200,8 +200,8 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     using A = typename Arith<S>::T;
     // psi tile rows of N + VE elements: 16-byte aligned, and consecutive rows
     // 4 banks apart, so the packet accesses below are conflict-free
-    constexpr int VE = Pk<S>::n, KV = 16 / VE;
-    constexpr int M = 16, P = N / M, WG = L * P, LF = L / 2, RS = N + VE;
+    constexpr int M = 16, VE = Pk<S>::n, KV = M / VE;
+    constexpr int P = N / M, WG = L * P, LF = L / 2, RS = N + VE;
     using Pl = FftPlan<N>;
     static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
     using F = FusedZ<N>;
