@@ -258,7 +258,11 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
         }
         const A hl = (act && e0 > 0) ? to_a(row[e0 - 1]) : zero;
         const A hr = (act && e0 + M < N) ? to_a(row[e0 + M]) : zero;
-        const double acc = seg_solve<A, M, COLS, true>(vv, hl, hr, e0, N, s, l, L, P, sh, k, wz);
+        // batched carry folds for real lines (measured 1.3 us faster at 256 in
+        // both precisions); complex lines keep the loop (slower batched: the
+        // extra registers cost occupancy)
+        constexpr int PC = std::is_floating_point<A>::value ? P : 0;
+        const double acc = seg_solve<A, M, COLS, true, PC>(vv, hl, hr, e0, N, s, l, L, P, sh, k, wz);
         if (act) {
             S *orow = tile + l * RS;
 #pragma unroll

@@ -181,7 +181,10 @@ __device__ __forceinline__ cpx<R> sweep_back(const SweepConst<cpx<R>> &k, cpx<R>
 // every iteration keeps the test.  The x / y sweeps of complex lines use the
 // latter: peeled, the compiler schedules them with more registers, which
 // measured slower there (and faster in every other sweep).
-template <typename A, int M, bool COLS, bool PEEL>
+// PC > 0: the line has exactly PC segments, known at compile time (P == PC);
+// the carry folds then load every segment's carry at once and run the same
+// FMA chain predicated, instead of one LDS round trip per step.
+template <typename A, int M, bool COLS, bool PEEL, int PC = 0>
 __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n, int s, int l, int L, int P,
                                             const SweepShared<A> &sh, const SweepConst<A> &k,
                                             const double *__restrict__ wz) {
@@ -214,7 +217,19 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
     __syncthreads();
     A x = sh.exl[l];
-    for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
+    if constexpr (PC > 0) {
+        A eb[PC], ea[PC];
+#pragma unroll
+        for (int t = 0; t < PC; ++t) {
+            eb[t] = sh.exb[t * L + l];
+            ea[t] = sh.exa[t * L + l];
+        }
+#pragma unroll
+        for (int t = PC - 1; t > 0; --t)
+            if (t > s) x = m_fma(eb[t], x, ea[t]);
+    } else {
+        for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
+    }
     {
         auto fixb = [&](int j) { v[j] = m_fma(sh.gb[e0 + j], x, v[j]); };
         if (inM) fixb(M - 1);
@@ -240,7 +255,19 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     sh.exb[s * L + l] = sh.fa[top];  // the segment's product of calpha
     __syncthreads();
     A y = zero;  // psi'[0] = 0
-    for (int t = 0; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
+    if constexpr (PC > 0) {
+        A eb[PC], ea[PC];
+#pragma unroll
+        for (int t = 0; t < PC - 1; ++t) {
+            eb[t] = sh.exb[t * L + l];
+            ea[t] = sh.exa[t * L + l];
+        }
+#pragma unroll
+        for (int t = 0; t < PC - 1; ++t)
+            if (t < s) y = m_fma(eb[t], y, ea[t]);
+    } else {
+        for (int t = 0; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
+    }
     double acc = 0.0;
     {
         auto fixf = [&](int j) {
