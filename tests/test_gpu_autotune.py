@@ -63,3 +63,39 @@ def test_tuned_choices_leave_results_bit_identical():
     assert run({"DBEC_AUTOTUNE": "0", **FORCED["one"]}) == rules
     assert run({"DBEC_AUTOTUNE": "0", **FORCED["two"]}) == rules
     assert run({}) == rules
+
+
+# The FP64 fused inverse z pass with real psi has two LDS layouts (padded and
+# XOR-swizzled); each z length has its own radix plan, and the plans' first
+# radices (4, 8, 16) set the swizzle.
+LAYOUT_SCRIPT = r"""
+import hashlib, json, sys
+sys.path.insert(0, sys.argv[1])
+import dbecpy
+lib = dbecpy.load(dbecpy.HIP_LIB)
+out = {}
+for nz in (48, 80, 96, 192, 256, 384, 512):
+    p = dbecpy.make_params(32, 48, nz, 0.25, 0.25, 0.1, 0.002, 0.5, 1.0, 1.5, 6.0)
+    with dbecpy.Solver(lib, p, dbecpy.IMAG, dbecpy.FP64) as s:
+        s.set_psi(dbecpy.gaussian_psi0(p))
+        s.calcnorm(rescale=True)
+        s.set_coupling(2 * 20.0, 2 * 10.0)
+        s.step(4)
+        out[str(nz)] = hashlib.sha256(s.get_psi().tobytes()).hexdigest()
+print(json.dumps(out))
+"""
+
+
+def test_fused_inverse_layouts_bit_identical():
+    def layout(v):
+        env = {k: x for k, x in os.environ.items()
+               if not k.startswith(("DBEC_FFT_LM_", "DBEC_FFT_ZINV_SWZ", "DBEC_SWEEP_", "DBEC_AUTOTUNE"))}
+        env.update({"DBEC_AUTOTUNE": "0", "DBEC_FFT_ZINV_SWZ": v})
+        print(f"[gpu-case] fused inverse layout {v}", flush=True)
+        r = subprocess.run([sys.executable, "-c", LAYOUT_SCRIPT, os.path.join(PKG, "python")], env=env,
+                           capture_output=True, text=True, timeout=180)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    padded = layout("0")
+    assert len(padded) == 7
+    assert layout("1") == padded
