@@ -388,7 +388,14 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     const int tid = threadIdx.x;
     const int ytiles = (Ny + L - 1) / L;
     const int kz = blockIdx.x / ytiles;
-    const int y0 = (blockIdx.x % ytiles) * L;
+    // y tiles of one kz: the first half in order, the second half reversed,
+    // so tile t and its mirror tile (ytiles - 1 - t, the same |ky| table
+    // rows) are h workgroups apart.  Workgroups go to the 8 XCDs round
+    // robin; with h a multiple of 8 (256 lines in tiles of 16) the pair runs
+    // on one XCD at about the same time and the second reads the table rows
+    // from that XCD's L2.
+    const int p = blockIdx.x % ytiles, h = (ytiles + 1) / 2;
+    const int y0 = (p < h ? p : ytiles - 1 - (p - h)) * L;
     const int nl = min(L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
     // Thread (l, t) holds x = t + k T of y line l, lanes along y (coalesced
