@@ -6,7 +6,10 @@
                                --workload W [--copy-mib M] [--cmd C]
 
 `stats` prints a Markdown table of per-kernel calls and durations (from
-`rocprofv3 --kernel-trace --stats --output-format csv`).  `traffic` turns two
+`rocprofv3 --kernel-trace --stats --output-format csv`).  A kernel that ran
+as several template variants (the autotuner's trial launches) is reported,
+in both commands, as the variant with the most dispatches: the one the step
+used.  `traffic` turns two
 `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs into HBM bytes per
 launch for each kernel, JSON on stdout.  Both counters are in KiB per
 dispatch; each is scaled by what it reads for the runtime's device-to-device
@@ -61,15 +64,24 @@ def short_name(full):
     }.get(k, k[2:])
 
 
+def dominant(variants):
+    """The variant (full kernel name) with the most dispatches.  A kernel that
+    ran as several template instantiations (the autotuner's trial launches at
+    context creation) is reported as the one the step used."""
+    return max(variants.items(), key=lambda kv: kv[1][0])
+
+
 def cmd_stats(args):
     rows = list(csv.DictReader(open(args.csv)))
-    agg = defaultdict(lambda: [0, 0.0, None])
+    byn = defaultdict(dict)
     for r in rows:
         n = short_name(r["Name"]) or r["Name"][:40]
-        a = agg[n]
-        a[0] += int(r["Calls"])
-        a[1] += float(r["TotalDurationNs"])
-        a[2] = r["Name"]
+        c, t = byn[n].get(r["Name"], (0, 0.0))
+        byn[n][r["Name"]] = (c + int(r["Calls"]), t + float(r["TotalDurationNs"]))
+    agg = {}
+    for n, variants in byn.items():
+        full, (calls, tot) = dominant(variants)
+        agg[n] = [calls, tot, full]
     total = sum(a[1] for a in agg.values())
     print(f"# {args.title}\n")
     if args.cmd:
@@ -103,15 +115,15 @@ def cmd_traffic(args):
     ff, fw = factor(fetch), factor(write)
     out = {"workload": args.workload, "command": args.cmd, "copy_calibration_mib": mib,
            "fetch_factor": ff, "write_factor": fw, "kernels": {}}
-    by_short = defaultdict(lambda: {"f": [], "w": []})
+    variants = defaultdict(dict)  # short name -> full name -> (dispatches,)
     for k, vs in fetch.items():
         s = short_name(k)
         if s and s != "copy":
-            by_short[s]["f"] += vs
-    for k, vs in write.items():
-        s = short_name(k)
-        if s and s != "copy":
-            by_short[s]["w"] += vs
+            variants[s][k] = (len(vs),)
+    by_short = {}
+    for s, vv in variants.items():
+        full, _ = dominant(vv)
+        by_short[s] = {"f": fetch.get(full, []), "w": write.get(full, [])}
     for s, d in sorted(by_short.items()):
         if not d["f"] or not d["w"]:
             continue
