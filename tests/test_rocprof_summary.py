@@ -1,0 +1,60 @@
+"""The rocprofv3 summary tool (python/rocprof_summary.py) on synthetic CSVs:
+kernel names map to the bench's short names, a kernel that ran as several
+template variants is reported as the variant with the most dispatches, and
+counter bytes are scaled by the runtime copy kernel of known size."""
+import csv
+import json
+import os
+import subprocess
+import sys
+
+from conftest import PKG
+
+TOOL = os.path.join(PKG, "python", "rocprof_summary.py")
+XC1 = "void dbec::k_xconv<256, float, 1>(dbec::Solves, float const*, int, dbec::cpx<float> const*)"
+XC2 = "void dbec::k_xconv<256, float, 2>(dbec::Solves, float const*, int, dbec::cpx<float> const*)"
+YF = "void dbec::k_ypass<256, -1, float, 1>(dbec::Solves, long, dbec::cpx<float> const*, double const*, int, int)"
+COPY = "__amd_rocclr_copyBuffer"
+
+
+def run(args):
+    r = subprocess.run([sys.executable, TOOL] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_stats_reports_the_dominant_variant(tmp_path):
+    p = tmp_path / "stats.csv"
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Name", "Calls", "TotalDurationNs", "AverageNs"])
+        w.writeheader()
+        w.writerow({"Name": XC1, "Calls": 7, "TotalDurationNs": 7 * 40000, "AverageNs": 40000})
+        w.writerow({"Name": XC2, "Calls": 200, "TotalDurationNs": 200 * 28000, "AverageNs": 28000})
+        w.writerow({"Name": YF, "Calls": 200, "TotalDurationNs": 200 * 22000, "AverageNs": 22000})
+    out = run(["stats", str(p), "--title", "t"])
+    rows = {l.split("|")[1].strip(): l for l in out.splitlines() if l.startswith("| ") and "---" not in l}
+    assert "| 200 | 28.0 |" in rows["xconv"] and "float, 2>" in rows["xconv"]
+    assert "| 200 | 22.0 |" in rows["ypass_fwd"]
+
+
+def test_traffic_scales_by_the_copy_and_keeps_the_dominant_variant(tmp_path):
+    def counters(path, name, vals):
+        with open(path, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
+            w.writeheader()
+            d = 0
+            for k, v, n in vals:
+                for _ in range(n):
+                    d += 1
+                    w.writerow({"Dispatch_Id": d, "Kernel_Name": k, "Counter_Name": name, "Counter_Value": v})
+    # the copy of 512 MiB reads as half that in FETCH_SIZE KiB (factor 2), in full in WRITE_SIZE
+    counters(tmp_path / "f.csv", "FETCH_SIZE", [(COPY, 256 * 1024, 1), (XC1, 140000, 7), (XC2, 50000, 21), (YF, 66000, 14)])
+    counters(tmp_path / "w.csv", "WRITE_SIZE", [(COPY, 512 * 1024, 1), (XC1, 66000, 7), (XC2, 66000, 21), (YF, 66000, 14)])
+    out = json.loads(run(["traffic", str(tmp_path / "f.csv"), str(tmp_path / "w.csv"), "--workload", "w",
+                          "--copy-mib", "512"]))
+    assert abs(out["fetch_factor"] - 2.0) < 1e-12 and abs(out["write_factor"] - 1.0) < 1e-12
+    xc = out["kernels"]["xconv"]
+    assert xc["dispatches"] == 21
+    assert xc["fetch_bytes"] == 2.0 * 1024 * 50000
+    assert xc["write_bytes"] == 1024 * 66000
+    assert out["kernels"]["ypass_fwd"]["fetch_bytes"] == 2.0 * 1024 * 66000
