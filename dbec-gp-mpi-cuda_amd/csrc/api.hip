@@ -2,6 +2,7 @@
 //
 // Reference aliases: imag3d.cu = src/imag3d-mpicuda/imag3d-mpicuda.cu,
 // real3d.cu = src/real3d-mpicuda/real3d-mpicuda.cu.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -325,7 +326,7 @@ Cx cx_div(Cx a, Cx b) {
 
 void free_buffers(dbec_ctx *c) {
     Buffers &b = c->b;
-    void *ptrs[] = {b.psi, b.spec, b.phi, b.potddq, b.rows, b.red, b.scal, b.coef, b.coef_lo, b.wgt, b.sq,
+    void *ptrs[] = {b.psi, b.spec, b.phi, b.potddq, b.rows, b.red, b.scal, b.coef, b.wgt, b.sq,
                     b.tw[0], b.tw[1], b.tw[2], b.twf[0], b.twf[1], b.twf[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -484,17 +485,12 @@ int setup(dbec_ctx *c) {
     CHK(dev_alloc(c, (void **)&b.rows, (size_t)2 * Nx * Ny * sizeof(double), "row sums"));
     CHK(dev_alloc(c, (void **)&b.scal, SC_COUNT * sizeof(double), "scalars"));
     CHK(dev_alloc(c, (void **)&b.coef, coef.size() * sizeof(double), "CN coefficients"));
-    std::vector<float> coef_f(coef.begin(), coef.end());  // FP32 tables of the mixed form's sweeps
-    if (f32) CHK(dev_alloc(c, &b.coef_lo, coef_f.size() * sizeof(float), "CN coefficients (FP32)"));
     CHK(dev_alloc(c, (void **)&b.wgt, wgt.size() * sizeof(double), "Simpson weights"));
     CHK(dev_alloc(c, (void **)&b.sq, sq.size() * sizeof(double), "grid squares"));
     hipStream_t s = c->stream;
     HIPCHK(c, hipMemsetAsync(b.psi, 0, (size_t)c->N * psi_elem(c), s), "memset");
     HIPCHK(c, hipMemsetAsync(b.phi, 0, (size_t)c->N * re, s), "memset");
     HIPCHK(c, hipMemcpyAsync(b.coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
-    if (f32)
-        HIPCHK(c, hipMemcpyAsync(b.coef_lo, coef_f.data(), coef_f.size() * sizeof(float), hipMemcpyHostToDevice, s),
-               "copy");
     HIPCHK(c, hipMemcpyAsync(b.wgt, wgt.data(), wgt.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
     HIPCHK(c, hipMemcpyAsync(b.sq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice, s), "copy");
     double sc[SC_COUNT] = {0};
@@ -527,9 +523,15 @@ int setup(dbec_ctx *c) {
 // times on the context's stream (psi and the spectrum are zero: the kernels
 // move the same bytes and do the same arithmetic as on a state) and the
 // fastest is kept, pass by pass.  The choices never change a result, only
-// its speed.  Results are kept per (grid, mode, precision, device) for the
+// its speed.  The built-in rule (Tuning's 0 / -1) is always a candidate and
+// is kept unless another value is faster by more than `margin`, so timing
+// noise cannot move a pass off the rule.  Candidates are timed in
+// interleaved rounds (rule, a, b, rule, a, b, ...) and compared by their
+// median round, so a clock ramp at context creation affects all of them
+// alike.  Results are kept per (grid, mode, precision, device) for the
 // process, so contexts of one shape share them.  DBEC_AUTOTUNE=0 keeps the
-// built-in rules; DBEC_AUTOTUNE_LOG=1 prints the choices to stderr.
+// built-in rules; DBEC_AUTOTUNE_LOG=1 prints the choices to stderr;
+// dbec_tuning() returns them as text.
 struct TuneKey {
     long nx, ny, nz;
     int mode, prec, dev;
@@ -540,29 +542,64 @@ struct TuneKey {
 std::mutex g_tune_mu;
 std::map<TuneKey, Tuning> g_tune;
 
-// Best of `trials` timings of `reps` back-to-back calls of f, in ms.
-template <typename F>
-int time_launches(dbec_ctx *c, F &&f, double *ms) {
-    constexpr int trials = 2, reps = 3;
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0), "event");
-    HIPCHK(c, hipEventCreate(&e1), "event");
-    double best = 1e30;
-    hipError_t err = f();  // warm-up (first launch of the variant)
-    for (int t = 0; t < trials && err == hipSuccess; t++) {
-        err = hipEventRecord(e0, c->stream);
-        for (int r = 0; r < reps && err == hipSuccess; r++) err = f();
-        if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        float v = 0.f;
-        if (err == hipSuccess) err = hipEventElapsedTime(&v, e0, e1);
-        if (err == hipSuccess) best = std::min(best, (double)v / reps);
+constexpr int kTuneRounds = 7, kTuneReps = 3;
+constexpr double kTuneMargin = 0.02;
+
+// Median over kTuneRounds interleaved rounds of kTuneReps back-to-back calls
+// of set(i) + launch(), for every candidate i < ncand; ms[i] in ms per call.
+template <typename S, typename F>
+int time_candidates(dbec_ctx *c, int ncand, S &&set, F &&launch, std::vector<double> &ms) {
+    std::vector<hipEvent_t> ev(2 * ncand * kTuneRounds, nullptr);
+    hipError_t err = hipSuccess;
+    for (auto &e : ev)
+        if (err == hipSuccess) err = hipEventCreate(&e);
+    for (int i = 0; i < ncand && err == hipSuccess; i++) {  // warm-up: first launch of each variant
+        set(i);
+        err = launch();
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (int r = 0; r < kTuneRounds && err == hipSuccess; r++)
+        for (int i = 0; i < ncand && err == hipSuccess; i++) {
+            set(i);
+            hipEvent_t a = ev[2 * (r * ncand + i)], b = ev[2 * (r * ncand + i) + 1];
+            err = hipEventRecord(a, c->stream);
+            for (int k = 0; k < kTuneReps && err == hipSuccess; k++) err = launch();
+            if (err == hipSuccess) err = hipEventRecord(b, c->stream);
+        }
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    ms.assign(ncand, 0.);
+    for (int i = 0; i < ncand && err == hipSuccess; i++) {
+        std::vector<double> t(kTuneRounds);
+        for (int r = 0; r < kTuneRounds && err == hipSuccess; r++) {
+            float v = 0.f;
+            err = hipEventElapsedTime(&v, ev[2 * (r * ncand + i)], ev[2 * (r * ncand + i) + 1]);
+            t[r] = (double)v / kTuneReps;
+        }
+        std::sort(t.begin(), t.end());
+        ms[i] = t[kTuneRounds / 2];
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
     HIPCHK(c, err, "autotune launch");
-    *ms = best;
     return DBEC_OK;
+}
+
+// Index of the candidate to keep: 0 is the built-in rule, kept unless another
+// is faster by more than kTuneMargin.
+int pick_candidate(const std::vector<double> &ms) {
+    int best = 0;
+    for (int i = 1; i < (int)ms.size(); i++)
+        if (ms[i] < ms[best]) best = i;
+    return (best != 0 && ms[best] < (1.0 - kTuneMargin) * ms[0]) ? best : 0;
+}
+
+void describe_tuning(const dbec_ctx *c, char *buf, size_t len) {
+    const Tuning &t = c->tune;
+    std::snprintf(buf, len,
+                  "%ldx%ldx%ld %s %s: lm zfwd %d ypass %d xconv %d fused %d; sweep x rowb %d peel %d, "
+                  "y rowb %d peel %d, z direct %d; fused layout %d (0 / -1 = built-in rule)",
+                  c->Nx, c->Ny, c->Nz, c->mode == DBEC_IMAG ? "imag" : "real",
+                  c->prec == DBEC_MIXED ? "mixed" : "fp64", t.lm[0], t.lm[1], t.lm[2], t.lm[3], t.xy_rowb[0],
+                  t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1], t.z_direct, t.zinv_swz);
 }
 
 int autotune(dbec_ctx *c) {
@@ -578,61 +615,43 @@ int autotune(dbec_ctx *c) {
         }
     }
     const Solves v = solve_ref(c, false);
-    // pick the fastest value of one field among the candidates
-    auto choose = [&](int *field, std::initializer_list<int> cands, auto &&launch) -> int {
-        double best = 1e30;
-        int pick = *field;
-        for (int cand : cands) {
-            *field = cand;
-            double ms;
-            CHK(time_launches(c, launch, &ms));
-            if (ms < best) {
-                best = ms;
-                pick = cand;
-            }
-        }
-        *field = pick;
+    // one field: candidate 0 is its rule value
+    auto choose = [&](int *field, std::vector<int> cands, auto &&launch) -> int {
+        std::vector<double> ms;
+        CHK(time_candidates(c, (int)cands.size(), [&](int i) { *field = cands[i]; }, launch, ms));
+        *field = cands[pick_candidate(ms)];
         return DBEC_OK;
     };
     Tuning &t = c->tune;
-    CHK(choose(&t.lm[0], {1, 2}, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }));
-    CHK(choose(&t.lm[1], {1, 2}, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }));
-    CHK(choose(&t.lm[2], {1, 2}, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }));
+    CHK(choose(&t.lm[0], {0, 1, 2}, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }));
+    CHK(choose(&t.lm[1], {0, 1, 2}, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }));
+    CHK(choose(&t.lm[2], {0, 1, 2}, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }));
     if (c->mode == DBEC_IMAG && c->prec != DBEC_MIXED) {
         // the fused inverse pass has two buffer layouts here: its tile
         // multiplier and layout are chosen together
-        double best = 1e30;
-        int blm = 1, bsw = 0;
-        for (int sw : {0, 1})
-            for (int lm : {1, 2}) {
-                t.zinv_swz = sw;
-                t.lm[3] = lm;
-                double ms;
-                CHK(time_launches(c, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }, &ms));
-                if (ms < best) {
-                    best = ms;
-                    blm = lm;
-                    bsw = sw;
-                }
-            }
-        t.lm[3] = blm;
-        t.zinv_swz = bsw;
+        const int lms[5] = {0, 1, 2, 1, 2}, sws[5] = {-1, 0, 0, 1, 1};
+        std::vector<double> ms;
+        CHK(time_candidates(
+            c, 5, [&](int i) { t.lm[3] = lms[i]; t.zinv_swz = sws[i]; },
+            [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }, ms));
+        const int i = pick_candidate(ms);
+        t.lm[3] = lms[i];
+        t.zinv_swz = sws[i];
     } else {
-        CHK(choose(&t.lm[3], {1, 2}, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }));
+        CHK(choose(&t.lm[3], {0, 1, 2}, [&] { return launch_zinv_nu(c->Nz, fft_args(c, 2, v)); }));
     }
     for (int axis = 0; axis < 2; axis++) {
         auto sweep = [&] { return launch_sweep(c->stream, c, axis, v, false); };
-        CHK(choose(&t.xy_peel[axis], {0, 1}, sweep));
-        CHK(choose(&t.xy_rowb[axis], {128, 256}, sweep));
+        CHK(choose(&t.xy_peel[axis], {-1, 0, 1}, sweep));
+        CHK(choose(&t.xy_rowb[axis], {0, 128, 256}, sweep));
     }
-    CHK(choose(&t.z_direct, {0, 1}, [&] { return launch_sweep(c->stream, c, 2, v, c->mode == DBEC_IMAG); }));
+    CHK(choose(&t.z_direct, {-1, 0, 1}, [&] { return launch_sweep(c->stream, c, 2, v, c->mode == DBEC_IMAG); }));
     // the candidates ran on zero data; psi and the spectrum stay zero
-    if (const char *lg = getenv("DBEC_AUTOTUNE_LOG"); lg && std::strcmp(lg, "0") != 0)
-        std::fprintf(stderr, "[dbec autotune] %ldx%ldx%ld %s %s: lm zfwd %d ypass %d xconv %d fused %d; "
-                             "sweep x rowb %d peel %d, y rowb %d peel %d, z direct %d; fused layout %d\n",
-                     c->Nx, c->Ny, c->Nz, c->mode == DBEC_IMAG ? "imag" : "real",
-                     c->prec == DBEC_MIXED ? "mixed" : "fp64", t.lm[0], t.lm[1], t.lm[2], t.lm[3], t.xy_rowb[0],
-                     t.xy_peel[0], t.xy_rowb[1], t.xy_peel[1], t.z_direct, t.zinv_swz);
+    if (const char *lg = getenv("DBEC_AUTOTUNE_LOG"); lg && std::strcmp(lg, "0") != 0) {
+        char buf[256];
+        describe_tuning(c, buf, sizeof buf);
+        std::fprintf(stderr, "[dbec autotune] %s\n", buf);
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = t;
     return DBEC_OK;
@@ -1031,6 +1050,12 @@ int dbec_get_table(dbec_ctx *c, int which, int axis, double *out) {
     default:
         return fail(c, DBEC_EINVAL, "bad table");
     }
+}
+
+int dbec_tuning(const dbec_ctx *c, char *buf, long len) {
+    if (!c || !buf || len < 1) return DBEC_EINVAL;
+    describe_tuning(c, buf, (size_t)len);
+    return DBEC_OK;
 }
 
 int dbec_synchronize(dbec_ctx *c) {

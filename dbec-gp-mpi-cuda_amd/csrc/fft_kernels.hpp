@@ -804,8 +804,9 @@ struct FftLaunchImpl {
         } else {
             const int yt = (int)((a.Ny + L - 1) / L);
             dim3 grid((unsigned)(a.Nx * yt), (unsigned)a.v.nb), block(L * (N / 16));
-            const SweepTab<A> tab{static_cast<const A *>(a.ztab[0]), static_cast<const A *>(a.ztab[1]),
-                                  static_cast<const A *>(a.ztab[2]), static_cast<const A *>(a.ztab[3])};
+            using TT = typename TabOf<A>::T;
+            const SweepTab<A> tab{static_cast<const TT *>(a.ztab[0]), static_cast<const TT *>(a.ztab[1]),
+                                  static_cast<const TT *>(a.ztab[2]), static_cast<const TT *>(a.ztab[3])};
             SweepConst<A> k;
             zconst(a, k);
             k_sweepz_zfwd<N, S, R, L, COLS><<<grid, block, 0, a.stream>>>(a.v, (int)a.Nx, (int)a.Ny, tab, k, a.wz, a.wy,

@@ -33,7 +33,7 @@ struct Buffers {
     double *red = nullptr;   // reduction scratch (>= 4 * (Nx + Ny + Nz) doubles)
     double *scal = nullptr;  // device scalars, see SC_* below
     double *coef = nullptr;  // CN coefficients and carry multipliers (CoefLayout), real or complex, FP64
-    void *coef_lo = nullptr; // same in FP32 for the mixed form
+                             // in both precisions (the mixed sweeps round them to FP32 in LDS)
     double *wgt = nullptr;   // Simpson weights wx | wy | wz (include h/3)
     double *sq = nullptr;    // x^2 | y^2 | z^2
     void *tw[3] = {nullptr, nullptr, nullptr};  // twiddles for Nx, Ny, Nz (FP64)

@@ -375,9 +375,9 @@ int sweep_z_rows(const dbec_ctx *c) {
 
 void sweep_z_args(const dbec_ctx *c, FftArgs &a) {
     const CoefLayout lay(c->Nx, c->Ny, c->Nz);
-    const bool f32 = c->prec == DBEC_MIXED, cplx = c->mode == DBEC_REAL;
-    const size_t el = (f32 ? sizeof(float) : sizeof(double)) * (cplx ? 2 : 1);
-    const char *base = static_cast<const char *>(f32 ? c->b.coef_lo : static_cast<const void *>(c->b.coef));
+    const bool cplx = c->mode == DBEC_REAL;
+    const size_t el = sizeof(double) * (cplx ? 2 : 1);  // FP64 tables in every precision
+    const char *base = reinterpret_cast<const char *>(c->b.coef);
     a.ztab[0] = base + el * lay.ca(2);
     a.ztab[1] = base + el * lay.cg(2);
     a.ztab[2] = base + el * lay.gb(2, 0);
@@ -409,7 +409,7 @@ struct Launch {
 // The tables of one axis for both segment lengths (16 and 32).
 template <typename A>
 struct SweepTabs {
-    const A *ca, *cg, *gb[2], *fa[2];
+    const typename TabOf<A>::T *ca, *cg, *gb[2], *fa[2];
     SweepTab<A> at(int M) const {
         const int m = M == 32 ? 1 : 0;
         return {ca, cg, gb[m], fa[m]};
@@ -514,8 +514,8 @@ static hipError_t sweep_z_dispatch(const Launch &q, int Nx, int Ny, int Nz, cons
 }
 
 template <typename S, typename A>
-static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const A *base, const SweepConst<A> &k,
-                             bool cols) {
+static hipError_t sweep_axis(const Launch &q, const dbec_ctx *c, int axis, const typename TabOf<A>::T *base,
+                             const SweepConst<A> &k, bool cols) {
     const int Nx = (int)c->Nx, Ny = (int)c->Ny, Nz = (int)c->Nz;
     const CoefLayout lay(c->Nx, c->Ny, c->Nz);
     const SweepTabs<A> t{base + lay.ca(axis), base + lay.cg(axis),
@@ -538,7 +538,7 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves
             k.A0r = (float)c->A0r[axis][0];
             k.A = (float)c->A[axis][0];
             k.mA = -k.A;
-            return sweep_axis<float>(q, c, axis, static_cast<const float *>(b.coef_lo), k, cols);
+            return sweep_axis<float>(q, c, axis, b.coef, k, cols);
         }
         SweepConst<double> k;
         k.A0r = c->A0r[axis][0];
@@ -554,7 +554,7 @@ hipError_t launch_sweep(hipStream_t s, const dbec_ctx *c, int axis, const Solves
         k.A0r = {(float)c->A0r[axis][0], (float)c->A0r[axis][1]};
         k.A = {(float)c->A[axis][0], (float)c->A[axis][1]};
         k.mA = {(float)c->mA[axis][0], (float)c->mA[axis][1]};
-        return sweep_axis<cpx<float>>(q, c, axis, static_cast<const cpx<float> *>(b.coef_lo), k, cols);
+        return sweep_axis<cpx<float>>(q, c, axis, reinterpret_cast<const cpx<double> *>(b.coef), k, cols);
     }
     SweepConst<cpx<double>> k;
     k.A0r = {c->A0r[axis][0], c->A0r[axis][1]};

@@ -17,8 +17,8 @@ struct SweepConst {
     V A0r, A, mA;  // diagonal of the explicit half, off-diagonal, and its negative
 };
 
-// Layout of the Crank-Nicolson coefficient buffers (b.coef in FP64, b.coef_lo
-// in FP32, elements of the line's value type): calpha and cgamma of the axes
+// Layout of the Crank-Nicolson coefficient buffer (b.coef, FP64 real or
+// complex elements in both precisions): calpha and cgamma of the axes
 // x, y, z back to back (n - 1 each), then the carry multipliers gb and fa of
 // the partitioned sweeps (n each per axis) for segments of 16 and of 32:
 //   [calpha | cgamma | gb16 | fa16 | gb32 | fa32]

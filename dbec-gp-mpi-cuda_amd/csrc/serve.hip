@@ -87,6 +87,10 @@ struct Bucket {
     SolveParams *h_par = nullptr;
     double *h_scal = nullptr;
     hipEvent_t ev = nullptr;
+    // recorded on this bucket's stream after a widen that read another
+    // bucket's slot (migrate); that bucket's stream waits on it before the
+    // slot can be written again
+    hipEvent_t ev_mig = nullptr;
     bool inflight = false;
     // chunk graphs: check_every steps of nb solves, by nb (the slot list and
     // parameters are device data the graph reads, so any composition of nb
@@ -187,6 +191,7 @@ struct dbec_server {
         (void)hipHostFree(b->h_par);
         (void)hipHostFree(b->h_scal);
         if (b->ev) (void)hipEventDestroy(b->ev);
+        if (b->ev_mig) (void)hipEventDestroy(b->ev_mig);
         for (auto g : b->graphs)
             if (g) (void)hipGraphExecDestroy(g);
         (void)dbec_destroy(b->tab);
@@ -222,6 +227,7 @@ struct dbec_server {
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_par, cap * sizeof(SolveParams), 0);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_scal, (size_t)cap * b->scal_st, 0);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_mig, hipEventDisableTiming);
         if (e != hipSuccess) {
             release(b.get());
             return fail(std::string("bucket allocation: ") + hipGetErrorString(e));
@@ -318,6 +324,11 @@ struct dbec_server {
         if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SHIFT, 0.0);
         if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_CAPPED, 0.0);
         if (e == hipSuccess) e = launch_set_scalar(v->stream, v->b.scal + SC_SCALE, 1.0);
+        // slot s of the source bucket is free once this returns: a request
+        // admitted into it writes it on the source bucket's stream, which
+        // must first wait for the widen to have read it
+        if (e == hipSuccess) e = hipEventRecord(b->ev_mig, v->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(from->tab->stream, b->ev_mig, 0);
         if (e != hipSuccess) return fail(std::string("polish: ") + hipGetErrorString(e));
         std::unique_ptr<Job> j = std::move(from->job[s]);
         j->polishing = true;
@@ -413,11 +424,14 @@ struct dbec_server {
                     rc = core_step(t, k ? vf : v, fuse && k > 0, fuse && k + 1 < K);
                 if (imag && rc == DBEC_OK) rc = core_norm(t, v);
                 hipError_t e2 = hipStreamEndCapture(st, &g);
-                if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
                 if (e == hipSuccess) e = e2;
-                if (e == hipSuccess) e = hipGraphInstantiate(&b->graphs[na], g, nullptr, nullptr, 0);
-                if (g) (void)hipGraphDestroy(g);
-                if (e != hipSuccess) return fail(std::string("chunk graph: ") + hipGetErrorString(e));
+                if (rc == DBEC_OK && e == hipSuccess) e = hipGraphInstantiate(&b->graphs[na], g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);  // on every path: the exec graph keeps what it needs
+                if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+                if (e != hipSuccess) {
+                    b->graphs[na] = nullptr;
+                    return fail(std::string("chunk graph: ") + hipGetErrorString(e));
+                }
             }
             e = hipGraphLaunch(b->graphs[na], st);
             if (e != hipSuccess) return fail(std::string("chunk graph launch: ") + hipGetErrorString(e));

@@ -66,8 +66,17 @@ __device__ __forceinline__ cpx<float> m_zero(cpx<float>) { return {0.0f, 0.0f}; 
 __device__ __forceinline__ double m_mod2(cpx<float> a) { return (double)a.x * a.x + (double)a.y * a.y; }
 
 // Storage and arithmetic types: FP64 lines in FP64, FP32 lines (mixed form)
-// in FP32 with FP32 coefficient tables rounded from the FP64 ones.
+// in FP32.  The coefficient tables are FP64 in device memory for every line
+// type (TabOf); the mixed form's sweeps round each entry to FP32 once, as they
+// stage it into LDS.
 template <typename S> struct Arith { using T = S; };
+template <typename A> struct TabOf { using T = A; };
+template <> struct TabOf<float> { using T = double; };
+template <> struct TabOf<cpx<float>> { using T = cpx<double>; };
+__device__ __forceinline__ double coef_round(double, double v) { return v; }
+__device__ __forceinline__ float coef_round(float, double v) { return (float)v; }
+__device__ __forceinline__ cpx<double> coef_round(cpx<double>, cpx<double> v) { return v; }
+__device__ __forceinline__ cpx<float> coef_round(cpx<float>, cpx<double> v) { return {(float)v.x, (float)v.y}; }
 template <typename S> __device__ __forceinline__ S to_a(S v) { return v; }
 template <typename S> __device__ __forceinline__ S from_a(S v) { return v; }
 
@@ -83,10 +92,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 //   gb[i] = prod_{t = i .. top} cgamma[t] A,   fa[e] = prod_{t = bot .. e} calpha[t - 1]
 // over the in-range indices (1..n-2) of the segment holding i / e, top and
 // bot its last and first in-range index.  They are products of coefficients
-// only, formed on the host in FP64.
+// only, formed on the host in FP64 and kept in FP64 (TabOf<A>).
 template <typename A>
 struct SweepTab {
-    const A *ca, *cg, *gb, *fa;
+    const typename TabOf<A>::T *ca, *cg, *gb, *fa;
 };
 
 // LDS layout shared by the sweep kernels: the four tables (n each), the
@@ -122,7 +131,8 @@ struct CoefPre {
 template <typename A>
 __device__ __forceinline__ CoefPre<A> coef_pre(const SweepTab<A> &t, int n, int tid) {
     const int i = tid < n ? tid : 0, ic = i < n - 1 ? i : 0;
-    return {t.ca[ic], t.cg[ic], t.gb[i], t.fa[i]};
+    const A z{};
+    return {coef_round(z, t.ca[ic]), coef_round(z, t.cg[ic]), coef_round(z, t.gb[i]), coef_round(z, t.fa[i])};
 }
 template <typename A>
 __device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const SweepTab<A> &t, int n,
@@ -135,13 +145,14 @@ __device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre
         sh.gb[tid] = c.gb;
         sh.fa[tid] = c.fa;
     }
+    const A z{};
     for (int i = tid + nthreads; i < n; i += nthreads) {
         if (i < n - 1) {
-            sh.ca[i] = t.ca[i];
-            sh.cg[i] = t.cg[i];
+            sh.ca[i] = coef_round(z, t.ca[i]);
+            sh.cg[i] = coef_round(z, t.cg[i]);
         }
-        sh.gb[i] = t.gb[i];
-        sh.fa[i] = t.fa[i];
+        sh.gb[i] = coef_round(z, t.gb[i]);
+        sh.fa[i] = coef_round(z, t.fa[i]);
     }
 }
 

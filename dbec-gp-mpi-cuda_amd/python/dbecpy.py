@@ -28,7 +28,7 @@ SYMBOLS = [
     "dbec_set_coupling", "dbec_get_coupling", "dbec_calcpsidd2", "dbec_get_psidd2",
     "dbec_calcnu", "dbec_calclux", "dbec_calcluy", "dbec_calcluz", "dbec_calcnorm",
     "dbec_calcmuen", "dbec_calcrms", "dbec_psi_center", "dbec_step", "dbec_density_count",
-    "dbec_density", "dbec_get_table", "dbec_synchronize", "dbec_profile_step",
+    "dbec_density", "dbec_get_table", "dbec_synchronize", "dbec_profile_step", "dbec_tuning",
 ]
 
 
@@ -84,6 +84,7 @@ def load(path=HIP_LIB):
         "dbec_get_table": ([P, I, I, D], I),
         "dbec_synchronize": ([P], I),
         "dbec_profile_step": ([P, I, ctypes.POINTER(KernelStat), I, ctypes.POINTER(I)], I),
+        "dbec_tuning": ([P, ctypes.c_char_p, L], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -244,6 +245,12 @@ class Solver:
         a = np.empty(n, dtype=np.float64)
         self._chk(self.lib.dbec_get_table(self.h, which, axis, _dp(a)))
         return a
+
+    def tuning(self):
+        """The context's launch choices as text (dbec_tuning)."""
+        buf = ctypes.create_string_buffer(512)
+        self._chk(self.lib.dbec_tuning(self.h, buf, len(buf)))
+        return buf.value.decode()
 
     def profile(self, reps=10):
         stats = (KernelStat * 64)()

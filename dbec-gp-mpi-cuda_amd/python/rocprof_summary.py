@@ -115,25 +115,33 @@ def cmd_traffic(args):
     ff, fw = factor(fetch), factor(write)
     out = {"workload": args.workload, "command": args.cmd, "copy_calibration_mib": mib,
            "fetch_factor": ff, "write_factor": fw, "kernels": {}}
-    variants = defaultdict(dict)  # short name -> full name -> (dispatches,)
-    for k, vs in fetch.items():
-        s = short_name(k)
-        if s and s != "copy":
-            variants[s][k] = (len(vs),)
-    by_short = {}
-    for s, vv in variants.items():
-        full, _ = dominant(vv)
-        by_short[s] = {"f": fetch.get(full, []), "w": write.get(full, [])}
-    for s, d in sorted(by_short.items()):
-        if not d["f"] or not d["w"]:
+    def dominant_per_short(per):
+        variants = defaultdict(dict)  # short name -> full name -> (dispatches,)
+        for k, vs in per.items():
+            s = short_name(k)
+            if s and s != "copy":
+                variants[s][k] = (len(vs),)
+        return {s: dominant(vv)[0] for s, vv in variants.items()}
+
+    # the dominant variant is chosen per counter pass: the two passes are
+    # separate runs, and a tuned launch choice may differ between them
+    fdom, wdom = dominant_per_short(fetch), dominant_per_short(write)
+    for s in sorted(set(fdom) | set(wdom)):
+        if s not in fdom or s not in wdom:
+            print(f"warning: {s} appears in only one counter pass; not reported", file=sys.stderr)
             continue
-        out["kernels"][s] = {
-            "dispatches": len(d["f"]),
-            "fetch_kib_raw": sum(d["f"]) / len(d["f"]),
-            "write_kib_raw": sum(d["w"]) / len(d["w"]),
-            "fetch_bytes": ff * 1024.0 * sum(d["f"]) / len(d["f"]),
-            "write_bytes": fw * 1024.0 * sum(d["w"]) / len(d["w"]),
+        f, w = fetch[fdom[s]], write[wdom[s]]
+        rec = {
+            "dispatches": len(f),
+            "fetch_kib_raw": sum(f) / len(f),
+            "write_kib_raw": sum(w) / len(w),
+            "fetch_bytes": ff * 1024.0 * sum(f) / len(f),
+            "write_bytes": fw * 1024.0 * sum(w) / len(w),
         }
+        if fdom[s] != wdom[s]:
+            print(f"warning: {s} ran different variants in the two passes: {fdom[s]} / {wdom[s]}", file=sys.stderr)
+            rec["fetch_variant"], rec["write_variant"] = fdom[s], wdom[s]
+        out["kernels"][s] = rec
     json.dump(out, sys.stdout, indent=1)
     print()
 

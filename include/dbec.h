@@ -38,9 +38,12 @@ typedef struct dbec_ctx dbec_ctx;
 typedef enum { DBEC_IMAG = 0, DBEC_REAL = 1 } dbec_mode;
 
 /* DBEC_FP64: every operator in FP64 / complex128.
- * DBEC_MIXED: FFT convolution, calcnu and the sweeps store and compute in
- * FP32 / complex64; norm, mu, E and rms reductions, the renormalisation factor
- * and the Thomas coefficients stay FP64. */
+ * DBEC_MIXED: psi, the spectrum and the dipolar potential are stored in FP32 /
+ * complex64, and the FFT convolution, calcnu and the sweeps compute in FP32.
+ * FP64 stays for the norm, mu, E and rms reductions (FP64 accumulators), the
+ * renormalisation factor, and the Thomas coefficient tables (calpha, cgamma
+ * and the partitioned sweeps' carry multipliers), which live in device memory
+ * in FP64 and are rounded to FP32 once as a sweep stages them on chip. */
 typedef enum { DBEC_FP64 = 0, DBEC_MIXED = 1 } dbec_prec;
 
 /* Physical and numerical parameters of one solve.  Mirrors the keys read by
@@ -126,6 +129,13 @@ int dbec_density(dbec_ctx *ctx, int kind, long outstpx, long outstpy, long outst
 #define DBEC_TABLE_CALPHA 2
 #define DBEC_TABLE_CGAMMA 3
 int dbec_get_table(dbec_ctx *ctx, int which, int axis, double *out);
+
+/* The launch choices the context runs with (tile widths, row widths and
+ * boundary peeling of its kernels, picked by timing at dbec_create), as one
+ * line of text in buf (at most len bytes, NUL-terminated).  The choices never
+ * change a result.  No counterpart in the reference (cuFFT planned its own
+ * transforms, imag3d.cu:1037-1137). */
+int dbec_tuning(const dbec_ctx *ctx, char *buf, long len);
 
 /* Wait for all work queued on the context's stream. */
 int dbec_synchronize(dbec_ctx *ctx);

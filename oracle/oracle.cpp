@@ -866,6 +866,12 @@ int dbec_get_table(dbec_ctx *c, int which, int axis, double *out) {
 
 int dbec_synchronize(dbec_ctx *c) { return c ? DBEC_OK : DBEC_EINVAL; }
 
+int dbec_tuning(const dbec_ctx *c, char *buf, long len) {
+    if (!c || !buf || len < 1) return DBEC_EINVAL;
+    std::snprintf(buf, (size_t)len, "oracle: no launch choices");
+    return DBEC_OK;
+}
+
 int dbec_profile_step(dbec_ctx *c, int reps, dbec_kernel_stat *stats, int max, int *n) {
     (void)reps; (void)stats; (void)max;
     if (n) *n = 0;

@@ -58,3 +58,24 @@ def test_traffic_scales_by_the_copy_and_keeps_the_dominant_variant(tmp_path):
     assert xc["fetch_bytes"] == 2.0 * 1024 * 50000
     assert xc["write_bytes"] == 1024 * 66000
     assert out["kernels"]["ypass_fwd"]["fetch_bytes"] == 2.0 * 1024 * 66000
+
+
+def test_traffic_picks_the_dominant_variant_per_counter_pass(tmp_path):
+    """The FETCH and WRITE passes are separate runs; when their dominant
+    variants differ, each counter comes from its own pass's variant and the
+    record names both."""
+    def counters(path, name, vals):
+        with open(path, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
+            w.writeheader()
+            for d, (k, v) in enumerate((k, v) for k, v, n in vals for _ in range(n)):
+                w.writerow({"Dispatch_Id": d, "Kernel_Name": k, "Counter_Name": name, "Counter_Value": v})
+    counters(tmp_path / "f.csv", "FETCH_SIZE", [(COPY, 256 * 1024, 1), (XC1, 140000, 3), (XC2, 50000, 21)])
+    counters(tmp_path / "w.csv", "WRITE_SIZE", [(COPY, 512 * 1024, 1), (XC1, 70000, 21), (XC2, 66000, 3)])
+    r = subprocess.run([sys.executable, TOOL, "traffic", str(tmp_path / "f.csv"), str(tmp_path / "w.csv"),
+                        "--workload", "w"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    xc = json.loads(r.stdout)["kernels"]["xconv"]
+    assert xc["fetch_bytes"] == 2.0 * 1024 * 50000 and xc["write_bytes"] == 1024 * 70000
+    assert xc["fetch_variant"] == XC2 and xc["write_variant"] == XC1
+    assert "different variants" in r.stderr
