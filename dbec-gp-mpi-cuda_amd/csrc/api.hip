@@ -841,13 +841,13 @@ int dbec_calcnorm(dbec_ctx *c, double *norm, int rescale) {
     return DBEC_OK;
 }
 
-int dbec_calcmuen(dbec_ctx *c, double *mu, double *en) {
+static int calcmuen_api(dbec_ctx *c, double *mu, double *en, bool phase) {
     if (!c || !mu || !en) return fail(c, DBEC_EINVAL, "null argument");
     (void)hipSetDevice(c->device);
     CHK(materialize_scale(c));
     CHK(run_calcpsidd2(c));
     double *rmu = c->b.rows, *ren = c->b.rows + c->Nx * c->Ny;
-    HIPCHK(c, launch_rows_muen(c->stream, c, rmu, ren), "muen rows");
+    HIPCHK(c, launch_rows_muen(c->stream, c, rmu, ren, phase), "muen rows");
     HIPCHK(c, launch_finalize(c->stream, c, 2, rmu, ren), "finalize");
     double v[2];
     CHK(read_scalars(c, v, SC_MU, 2));
@@ -855,6 +855,9 @@ int dbec_calcmuen(dbec_ctx *c, double *mu, double *en) {
     *en = v[1];
     return DBEC_OK;
 }
+
+int dbec_calcmuen(dbec_ctx *c, double *mu, double *en) { return calcmuen_api(c, mu, en, false); }
+int dbec_calcmuen_phase(dbec_ctx *c, double *mu, double *en) { return calcmuen_api(c, mu, en, true); }
 
 int dbec_calcrms(dbec_ctx *c, double rms[4]) {
     if (!c || !rms) return fail(c, DBEC_EINVAL, "null argument");

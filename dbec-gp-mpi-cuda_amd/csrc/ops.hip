@@ -641,21 +641,50 @@ hipError_t launch_rows_mod2(hipStream_t s, const dbec_ctx *c, double *rows0, dou
     return hipGetLastError();
 }
 
+// Components a derivative is taken of: |psi| (the reference's calcmuen), or
+// the real or imaginary part (the phase-gradient energy of real time).
+enum { COMP_AMP = 0, COMP_RE = 1, COMP_IM = 2 };
+template <int C> __device__ __forceinline__ double comp_of(double v) { return v; }
+template <int C> __device__ __forceinline__ double comp_of(float v) { return (double)v; }
+template <int C> __device__ __forceinline__ double comp_of(cpx<double> v) {
+    return C == COMP_RE ? v.x : (C == COMP_IM ? v.y : amp_of(v));
+}
+template <int C> __device__ __forceinline__ double comp_of(cpx<float> v) {
+    return C == COMP_RE ? (double)v.x : (C == COMP_IM ? (double)v.y : amp_of(v));
+}
+
 // Five-point derivative with the boundary rule of calcmuen_kernel1x
 // (imag3d.cu:1303-1332): 0 at both ends, centred 3-point at 1 and N-2.
-template <typename P>
+template <int C = COMP_AMP, typename P>
 __device__ __forceinline__ double deriv5(const P *__restrict__ f, int i, int n, long st, double h) {
     if (i == 0 || i == n - 1) return 0.0;
-    if (i == 1) return (amp_of(f[2 * st]) - amp_of(f[0])) / (2.0 * h);
-    if (i == n - 2) return (amp_of(f[(long)(n - 1) * st]) - amp_of(f[(long)(n - 3) * st])) / (2.0 * h);
-    return (amp_of(f[(long)(i - 2) * st]) - 8.0 * amp_of(f[(long)(i - 1) * st]) + 8.0 * amp_of(f[(long)(i + 1) * st]) -
-            amp_of(f[(long)(i + 2) * st])) / (12.0 * h);
+    if (i == 1) return (comp_of<C>(f[2 * st]) - comp_of<C>(f[0])) / (2.0 * h);
+    if (i == n - 2) return (comp_of<C>(f[(long)(n - 1) * st]) - comp_of<C>(f[(long)(n - 3) * st])) / (2.0 * h);
+    return (comp_of<C>(f[(long)(i - 2) * st]) - 8.0 * comp_of<C>(f[(long)(i - 1) * st]) +
+            8.0 * comp_of<C>(f[(long)(i + 1) * st]) - comp_of<C>(f[(long)(i + 2) * st])) / (12.0 * h);
+}
+
+// Kinetic density sum over the axes of (d psi)^2 / (3 - par) at (i, j, k) of
+// the row `row`: of |psi|, or with PHASE of the real and imaginary parts.
+template <int C, typename P>
+__device__ __forceinline__ double kin_comp(const P *__restrict__ psi, int i, int j, int k, long row, int Nx, int Ny,
+                                           int Nz, const MuenConst &mc) {
+    const long NyNz = (long)Ny * Nz;
+    const double dxv = deriv5<C>(psi + (long)j * Nz + k, i, Nx, NyNz, mc.dx);
+    double d = (dxv * dxv) / mc.kin;
+    const double dyv = deriv5<C>(psi + (long)i * NyNz + k, j, Ny, (long)Nz, mc.dy);
+    d += (dyv * dyv) / mc.kin;
+    const double dzv = deriv5<C>(psi + row * Nz, k, Nz, 1L, mc.dz);
+    d += (dzv * dzv) / mc.kin;
+    return d;
 }
 
 // Per-row z integrals of the chemical potential and energy densities
 // (calcmuen_kernel1x/1y/1z and kernel2, imag3d.cu:1293-1450; real3d.cu
 // 1382-1543, where real time differentiates |psi|).
-template <typename P, typename R>
+// PHASE (complex psi): the kinetic density of the real and imaginary parts
+// (|grad psi|^2) instead of that of |psi|.
+template <typename P, typename R, bool PHASE = false>
 __global__ void __launch_bounds__(256)
 k_rows_muen(const P *__restrict__ psi, const R *__restrict__ phi, const double *__restrict__ sq,
             const double *__restrict__ wz, TrapConst tc, MuenConst mc, double *__restrict__ rmu,
@@ -664,16 +693,14 @@ k_rows_muen(const P *__restrict__ psi, const R *__restrict__ phi, const double *
     const long row = (long)blockIdx.x * 4 + wave;
     if (row >= (long)Nx * Ny) return;
     const int i = (int)(row / Ny), j = (int)(row % Ny);
-    const long NyNz = (long)Ny * Nz;
     double amu = 0.0, aen = 0.0;
     for (int k = lane; k < Nz; k += 64) {
         const long n = row * Nz + k;
-        const double dxv = deriv5(psi + (long)j * Nz + k, i, Nx, NyNz, mc.dx);
-        double d = (dxv * dxv) / mc.kin;
-        const double dyv = deriv5(psi + (long)i * NyNz + k, j, Ny, (long)Nz, mc.dy);
-        d += (dyv * dyv) / mc.kin;
-        const double dzv = deriv5(psi + row * Nz, k, Nz, 1L, mc.dz);
-        d += (dzv * dzv) / mc.kin;
+        double d;
+        if constexpr (PHASE)
+            d = kin_comp<COMP_RE>(psi, i, j, k, row, Nx, Ny, Nz, mc) + kin_comp<COMP_IM>(psi, i, j, k, row, Nx, Ny, Nz, mc);
+        else
+            d = kin_comp<COMP_AMP>(psi, i, j, k, row, Nx, Ny, Nz, mc);
         double psi2 = amp_of(psi[n]);
         psi2 *= psi2;
         const double psi2lin = psi2 * mc.g;
@@ -692,7 +719,7 @@ k_rows_muen(const P *__restrict__ psi, const R *__restrict__ phi, const double *
     }
 }
 
-hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren) {
+hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren, bool phase) {
     const auto &b = c->b;
     TrapConst tc;
     tc.half_par = 0.5 * c->p.par;
@@ -719,6 +746,15 @@ hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, doubl
             k_rows_muen<double, double><<<grid, block, 0, s>>>(static_cast<const double *>(b.psi),
                                                                static_cast<const double *>(b.phi), b.sq, wz, tc, mc, rmu,
                                                                ren, Nx, Ny, Nz);
+    } else if (phase) {
+        if (f32)
+            k_rows_muen<cpx<float>, float, true><<<grid, block, 0, s>>>(static_cast<const cpx<float> *>(b.psi),
+                                                                        static_cast<const float *>(b.phi), b.sq, wz,
+                                                                        tc, mc, rmu, ren, Nx, Ny, Nz);
+        else
+            k_rows_muen<cpx<double>, double, true><<<grid, block, 0, s>>>(static_cast<const cpx<double> *>(b.psi),
+                                                                          static_cast<const double *>(b.phi), b.sq,
+                                                                          wz, tc, mc, rmu, ren, Nx, Ny, Nz);
     } else {
         if (f32)
             k_rows_muen<cpx<float>, float><<<grid, block, 0, s>>>(static_cast<const cpx<float> *>(b.psi),

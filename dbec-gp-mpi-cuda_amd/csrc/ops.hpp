@@ -54,7 +54,8 @@ hipError_t launch_norm_from_cols(hipStream_t s, const dbec_ctx *c, const Solves 
 TrapConst trap_const(const dbec_ctx *c);
 NuConst nu_const(const dbec_ctx *c);
 hipError_t launch_rows_mod2(hipStream_t s, const dbec_ctx *c, double *rows0, double *rowsz);
-hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren);
+// phase: complex psi's kinetic density |grad psi|^2 instead of (grad |psi|)^2
+hipError_t launch_rows_muen(hipStream_t s, const dbec_ctx *c, double *rmu, double *ren, bool phase = false);
 // mode 0: norm and 1/norm; 1: norm and rms moments; 2: mu and E
 hipError_t launch_finalize(hipStream_t s, const dbec_ctx *c, int mode, const double *A, const double *B);
 hipError_t launch_reduce3(hipStream_t s, const dbec_ctx *c, int axis, double *out);

@@ -27,7 +27,7 @@ SYMBOLS = [
     "dbec_create", "dbec_destroy", "dbec_last_error", "dbec_set_psi", "dbec_get_psi",
     "dbec_set_coupling", "dbec_get_coupling", "dbec_calcpsidd2", "dbec_get_psidd2",
     "dbec_calcnu", "dbec_calclux", "dbec_calcluy", "dbec_calcluz", "dbec_calcnorm",
-    "dbec_calcmuen", "dbec_calcrms", "dbec_psi_center", "dbec_step", "dbec_density_count",
+    "dbec_calcmuen", "dbec_calcmuen_phase", "dbec_calcrms", "dbec_psi_center", "dbec_step", "dbec_density_count",
     "dbec_density", "dbec_get_table", "dbec_synchronize", "dbec_profile_step", "dbec_tuning",
 ]
 
@@ -76,6 +76,7 @@ def load(path=HIP_LIB):
         "dbec_calcluz": ([P], I),
         "dbec_calcnorm": ([P, D, I], I),
         "dbec_calcmuen": ([P, D, D], I),
+        "dbec_calcmuen_phase": ([P, D, D], I),
         "dbec_calcrms": ([P, D], I),
         "dbec_psi_center": ([P, D], I),
         "dbec_step": ([P, L, ctypes.c_double, ctypes.c_double, D], I),
@@ -209,6 +210,12 @@ class Solver:
     def calcmuen(self):
         mu, en = ctypes.c_double(), ctypes.c_double()
         self._chk(self.lib.dbec_calcmuen(self.h, ctypes.byref(mu), ctypes.byref(en)))
+        return mu.value, en.value
+
+    def calcmuen_phase(self):
+        """mu and E with the phase gradient in the kinetic term (dbec_calcmuen_phase)."""
+        mu, en = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.dbec_calcmuen_phase(self.h, ctypes.byref(mu), ctypes.byref(en)))
         return mu.value, en.value
 
     def calcrms(self):

@@ -93,6 +93,12 @@ int dbec_calcluz(dbec_ctx *ctx);             /* calcluz imag3d.cu:1907; real3d.c
  * *norm receives the norm before rescaling. */
 int dbec_calcnorm(dbec_ctx *ctx, double *norm, int rescale);
 int dbec_calcmuen(dbec_ctx *ctx, double *mu, double *en); /* calcmuen imag3d.cu:1250; real3d.cu:1339 (raw, not divided by par) */
+/* mu and E with the kinetic density |grad psi|^2 of the complex wave function
+ * (the phase gradient included): the energy a real-time run conserves.  The
+ * reference's real-time calcmuen differentiates |psi| instead
+ * (real3d.cu:1401-1424), so its E ignores the flow; this is an addition with
+ * no counterpart there.  Imaginary time: identical to dbec_calcmuen. */
+int dbec_calcmuen_phase(dbec_ctx *ctx, double *mu, double *en);
 int dbec_calcrms(dbec_ctx *ctx, double rms[4]);          /* calcrms imag3d.cu:1639; real3d.cu:1733: <r>, <x>, <y>, <z> */
 /* psi(Nx/2, Ny/2, Nz/2) as {re, im} (imag3d.cu:231, 244; real3d.cu:248). */
 int dbec_psi_center(dbec_ctx *ctx, double out[2]);

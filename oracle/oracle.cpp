@@ -421,28 +421,44 @@ double calcnorm_impl(dbec_ctx *c, int rescale) {
 }
 
 /* calcmuen (imag3d.cu:1250-1450; real3d.cu:1339-1543).  Real time
- * differentiates |psi| (real3d.cu:1401-1510). */
-void calcmuen_impl(dbec_ctx *c, double *mu, double *en) {
+ * differentiates |psi| (real3d.cu:1401-1510).  phase (an extension, real
+ * time only): the kinetic density is |grad psi|^2 of the complex psi, the
+ * same stencil applied to the real and imaginary parts, squares added. */
+void calcmuen_impl(dbec_ctx *c, double *mu, double *en, bool phase = false) {
     calcpsidd2_impl(c);
     const long Nx = c->Nx, Ny = c->Ny, Nz = c->Nz, N = c->N;
     const double par = c->p.par, Nd = (double)(Nx * Ny * Nz);
-    std::vector<double> f(N), dpsi(N), tmu(N), ten(N);
+    const bool cplx = phase && c->mode == DBEC_REAL;
+    std::vector<double> f(N), dpsi(N), tmu(N), ten(N), fi(cplx ? N : 0);
 #pragma omp parallel for schedule(static)
-    for (long n = 0; n < N; n++)
-        f[n] = (c->mode == DBEC_IMAG) ? c->psir[n] : cabs_(c->psic[n]);
+    for (long n = 0; n < N; n++) {
+        if (cplx) {
+            f[n] = c->psic[n].re;
+            fi[n] = c->psic[n].im;
+        } else {
+            f[n] = (c->mode == DBEC_IMAG) ? c->psir[n] : cabs_(c->psic[n]);
+        }
+    }
+    auto kin = [&](const std::vector<double> &g, long i, long j, long k) {
+        double dxv = deriv5(g.data() + idx(c, 0, j, k), i, Nx, Ny * Nz, c->p.dx);
+        double d = (dxv * dxv) / (3. - par);
+        double dyv = deriv5(g.data() + idx(c, i, 0, k), j, Ny, Nz, c->p.dy);
+        d += (dyv * dyv) / (3. - par);
+        double dzv = deriv5(g.data() + idx(c, i, j, 0), k, Nz, 1, c->p.dz);
+        d += (dzv * dzv) / (3. - par);
+        return d;
+    };
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < Nx; i++)
         for (long j = 0; j < Ny; j++)
             for (long k = 0; k < Nz; k++) {
                 long n = idx(c, i, j, k);
-                double dxv = deriv5(f.data() + idx(c, 0, j, k), i, Nx, Ny * Nz, c->p.dx);
-                double d = (dxv * dxv) / (3. - par);
-                double dyv = deriv5(f.data() + idx(c, i, 0, k), j, Ny, Nz, c->p.dy);
-                d += (dyv * dyv) / (3. - par);
-                double dzv = deriv5(f.data() + idx(c, i, j, 0), k, Nz, 1, c->p.dz);
-                d += (dzv * dzv) / (3. - par);
-                dpsi[n] = d;
+                dpsi[n] = cplx ? kin(f, i, j, k) + kin(fi, i, j, k) : kin(f, i, j, k);
             }
+    if (cplx) {
+#pragma omp parallel for schedule(static)
+        for (long n = 0; n < N; n++) f[n] = cabs_(c->psic[n]);
+    }
     const double g = c->g, gd = c->gd;
 #pragma omp parallel for schedule(static)
     for (long n = 0; n < N; n++) {
@@ -711,6 +727,12 @@ int dbec_calcnorm(dbec_ctx *c, double *norm, int rescale) {
 int dbec_calcmuen(dbec_ctx *c, double *mu, double *en) {
     if (!c || !mu || !en) return fail(c, DBEC_EINVAL, "null argument");
     calcmuen_impl(c, mu, en);
+    return DBEC_OK;
+}
+
+int dbec_calcmuen_phase(dbec_ctx *c, double *mu, double *en) {
+    if (!c || !mu || !en) return fail(c, DBEC_EINVAL, "null argument");
+    calcmuen_impl(c, mu, en, true);
     return DBEC_OK;
 }
 

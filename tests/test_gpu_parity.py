@@ -51,6 +51,7 @@ def test_operator_parity(hip_lib, oracle_lib, mode, n):
         assert rel(g.get_psi(), o.get_psi()) < TOL_OP, "calcnorm rescale"
         mg, mo = g.calcmuen(), o.calcmuen()
         assert mg == pytest.approx(mo, rel=TOL_OP)
+        assert g.calcmuen_phase() == pytest.approx(o.calcmuen_phase(), rel=TOL_OP)
         assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL_OP)
         assert g.psi_center() == pytest.approx(o.psi_center(), rel=TOL_OP, abs=1e-300)
         for kind in range(10):

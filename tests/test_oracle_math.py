@@ -186,3 +186,52 @@ def test_convolution_equals_direct_dft_on_tiny_grid(oracle_lib):
     Wi = np.conj(W)
     phi = np.einsum("ia,jb,kc,abc->ijk", Wi, Wi, Wi, F * full).real
     assert rel(a, face_copies(phi)) < 1e-13
+
+
+def test_phase_energy_pins(oracle_lib):
+    """calcmuen_phase: the kinetic density of the complex psi.
+    * imaginary time: identical to calcmuen;
+    * a positive psi times a constant phase e^{i theta}: the squares of the
+      real and imaginary parts' derivatives add up to those of |psi|, so it
+      equals calcmuen (which differentiates |psi|) to round-off;
+    * psi = phi e^{i k x} with phi > 0: the flow adds k^2/(3 - par) per unit
+      of |psi|^2, so E and mu grow by k^2 norm^2 / (3 - par), while calcmuen
+      does not see the flow.  The five-point stencil's product-rule error is
+      O(h^4) of phi's fifth derivative (here about 1e-3 of the flow term); it
+      scales as h^4, which a grid twice as fine shows."""
+    # boxes wide enough that phi is negligible where the end rules of the
+    # stencil (zero, then three-point) apply
+    p = params(n=(48, 40, 40), h=(0.3, 0.25, 0.2), cutoff=4.0)
+    phi = np.abs(dbecpy.gaussian_psi0(p))
+    with dbecpy.Solver(oracle_lib, p, dbecpy.IMAG) as s:
+        s.set_psi(phi)
+        s.set_coupling(30.0, 20.0)
+        assert s.calcmuen_phase() == s.calcmuen()
+    x = dbecpy.grid(p)[0][:, None, None]
+    with dbecpy.Solver(oracle_lib, p, dbecpy.REAL) as s:
+        s.set_coupling(30.0, 20.0)
+        s.set_psi(phi * np.exp(0.7j))
+        ref_mu, ref_en = s.calcmuen()
+        mu, en = s.calcmuen_phase()
+        assert mu == pytest.approx(ref_mu, rel=1e-13) and en == pytest.approx(ref_en, rel=1e-13)
+        k = 0.5
+        s.set_psi(phi * np.exp(1j * k * x))
+        assert s.calcmuen() == pytest.approx((ref_mu, ref_en), rel=1e-13)
+        norm2 = s.calcnorm() ** 2
+        mu, en = s.calcmuen_phase()
+        flow = k * k * norm2 / (3.0 - p.par)
+        err = abs((en - ref_en) - flow) / flow
+        assert err < 2e-3
+        assert mu - ref_mu == pytest.approx(en - ref_en, rel=1e-12)
+    # half the spacing: the discretisation error falls by about 2^4
+    p2 = params(n=(96, 80, 80), h=(0.15, 0.125, 0.1), cutoff=4.0)
+    phi2 = np.abs(dbecpy.gaussian_psi0(p2))
+    x2 = dbecpy.grid(p2)[0][:, None, None]
+    with dbecpy.Solver(oracle_lib, p2, dbecpy.REAL) as s:
+        s.set_coupling(30.0, 20.0)
+        s.set_psi(phi2)
+        e0 = s.calcmuen_phase()[1]
+        s.set_psi(phi2 * np.exp(1j * k * x2))
+        flow2 = k * k * s.calcnorm() ** 2 / (3.0 - p2.par)
+        err2 = abs((s.calcmuen_phase()[1] - e0) - flow2) / flow2
+    assert 8.0 < err / err2 < 32.0, (err, err2)
