@@ -30,6 +30,7 @@ sys.path.insert(0, os.path.join(PKG, "python"))
 
 METRIC = "grid-pt-steps/s at 256³ (FP64 vs mixed) + solves/s, p50/p99 at |Δμ/μ|≤1e-8"
 HBM_PEAK_GBS = 8000.0
+STREAM_FIXTURE = os.path.join(REPO, "tests", "golden", "serve_stream.json")
 WORKLOADS = {
     # name: (n, h, dt, cutoff, mode, prec)
     "imag256_fp64": ((256, 256, 256), 0.1, 0.0005, 12.8, "imag", "fp64"),
@@ -117,7 +118,9 @@ def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeou
     import dbecpy
     import dbecserve as ds
 
-    stream = ds.stream_requests(count, rate=rate, seed=0, prec=dbecpy.MIXED)
+    stream = ds.load_stream(STREAM_FIXTURE, count, prec=dbecpy.MIXED)
+    if rate != 20.0:  # the fixture's arrivals are at 20 requests/s
+        stream = [(t * 20.0 / rate, r) for t, r in stream]
     # capacity: the same requests all submitted at once
     with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every) as srv:
         t0 = time.perf_counter()

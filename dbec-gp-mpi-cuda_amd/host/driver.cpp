@@ -319,17 +319,21 @@ void observe(dbec_ctx *ctx, Obs &o, bool measure_norm) {
     o.c2 = a * a;
 }
 
-/* Extension: observables every OBSSTP steps. */
+/* Extension: observables every OBSSTP steps.  Real time adds the energy
+ * with the phase gradient in the kinetic term (dbec_calcmuen_phase), the one
+ * the dynamics conserves; E/par is the reference's, of |psi|. */
 struct ObsLog {
     FILE *f = nullptr;
-    void open(const Params &P, bool real) {
+    bool real = false;
+    void open(const Params &P, bool real_) {
+        real = real_;
         if (!P.obsout) return;
         char fn[512];
         std::snprintf(fn, sizeof fn, "%s.txt", P.obsout);
         f = std::fopen(fn, "w");
         if (!f) die("cannot open OBSOUT file\n");
-        std::fprintf(f, "# %s: step  t  norm  mu/par  E/par  <r>  <x>  <y>  <z>\n",
-                     real ? "real time" : "imaginary time");
+        std::fprintf(f, "# %s: step  t  norm  mu/par  E/par  <r>  <x>  <y>  <z>%s\n",
+                     real ? "real time" : "imaginary time", real ? "  Ephase/par" : "");
     }
     /* Measures the observables (norm measured, no rescale) and logs them;
      * returns mu. */
@@ -337,8 +341,14 @@ struct ObsLog {
         Obs o;
         observe(ctx, o, true);
         if (f) {
-            std::fprintf(f, "%ld %.10e %.16e %.16e %.16e %.16e %.16e %.16e %.16e\n", step, t, o.norm,
-                         o.mu / par, o.en / par, o.rms[0], o.rms[1], o.rms[2], o.rms[3]);
+            std::fprintf(f, "%ld %.10e %.16e %.16e %.16e %.16e %.16e %.16e %.16e", step, t, o.norm, o.mu / par,
+                         o.en / par, o.rms[0], o.rms[1], o.rms[2], o.rms[3]);
+            if (real) {
+                double mup, enp;
+                check(dbec_calcmuen_phase(ctx, &mup, &enp), ctx);
+                std::fprintf(f, " %.16e", enp / par);
+            }
+            std::fprintf(f, "\n");
             std::fflush(f);
         }
         return o.mu;

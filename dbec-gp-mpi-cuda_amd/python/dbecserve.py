@@ -3,11 +3,13 @@ mixed-size request stream used to measure it.
 
 The server batches solves of equal grid/dx/dt/cutoff/mode/precision (a size
 bucket) into shared launches and admits requests into free slots between
-chunks of steps.  `stream_requests` draws the serving stream: Poisson arrivals,
-cubic grids N in {48, 64, 96, 128, 160, 192} with dx = 16/N, G0 ~ U[5, 60],
-GDD0 ~ U[0, 40], gamma, nu, lambda ~ U[0.3, 3], 85 % ground states to
-|dmu/mu| < 1e-8 and 15 % real-time runs, from numpy's PCG64 generator with a
-fixed seed and a fixed order of draws.
+chunks of steps.  `load_stream` reads the serving stream from its JSON
+fixture (tests/golden/serve_stream.json, written once by
+tests/golden/make_serve_stream.py): Poisson arrivals at 20/s, cubic grids N in
+{48, 64, 96, 128, 160, 192} with dx = 16/N, G0 ~ U[5, 60], GDD0 ~ U[0, 40],
+gamma, nu, lambda ~ U[0.3, 3], 85 % ground states to |dmu/mu| < 1e-8 and
+15 % real-time runs, together with the settings the stream fixes (dt, cutoff,
+step counts).
 """
 import ctypes
 import os
@@ -142,28 +144,24 @@ class Server:
         self.close()
 
 
-SIZES = (48, 64, 96, 128, 160, 192)
+def load_stream(path, count=None, prec=dbecpy.MIXED):
+    """The serving stream of the JSON fixture at `path`: (arrival time,
+    Request) pairs in arrival order, the first `count` of them (all when
+    None), in precision `prec`.  Request ids are the stream positions."""
+    import json
 
-
-def stream_requests(count, rate=20.0, seed=0, prec=dbecpy.MIXED, dt_imag=0.01, dt_real=0.001, real_steps=5000,
-                    imag_cap=20000, mutol=1e-8):
-    """The serving stream: (arrival time, Request) pairs in arrival order.
-    Draw order per request: inter-arrival gap, size index, G0, GDD0, gamma,
-    nu, lambda, real-time flag."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    t, out = 0.0, []
-    for i in range(count):
-        t += rng.exponential(1.0 / rate)
-        n = SIZES[int(rng.integers(0, len(SIZES)))]
-        g0 = rng.uniform(5.0, 60.0)
-        gd0 = rng.uniform(0.0, 40.0)
-        trap = (rng.uniform(0.3, 3.0), rng.uniform(0.3, 3.0), rng.uniform(0.3, 3.0))
-        real = rng.uniform() < 0.15
+    d = json.load(open(path))
+    rows = d["requests"][: (count if count is not None else len(d["requests"]))]
+    out = []
+    for i, (t, n, g0, gd0, ga, nu, la, real) in enumerate(rows):
+        n = int(n)
         if real:
-            req = make_request(i, n, dbecpy.REAL, prec, g0, gd0, trap, dt_real, real_steps)
+            req = make_request(i, n, dbecpy.REAL, prec, g0, gd0, (ga, nu, la), d["dt_real"], d["real_steps"],
+                               cutoff=d["cutoff"], box=d["box"], par=d["par"])
         else:
-            req = make_request(i, n, dbecpy.IMAG, prec, g0, gd0, trap, dt_imag, imag_cap, mutol)
-        out.append((t, req))
+            req = make_request(i, n, dbecpy.IMAG, prec, g0, gd0, (ga, nu, la), d["dt_imag"], d["imag_cap"],
+                               mutol=d["mutol"], cutoff=d["cutoff"], box=d["box"], par=d["par"])
+        out.append((float(t), req))
     return out
 
 

@@ -148,35 +148,77 @@ def test_real_trajectory(hip_lib, oracle_lib):
             assert g.calcnorm() == pytest.approx(o.calcnorm(), rel=TOL)
 
 
-def test_imag_256_short(hip_lib, oracle_lib):
-    """The 256^3 workload (dx = 0.1, dt = 5e-4, R = 12.8), 8 steps."""
-    say("imag 256^3 x 8")
+# Couplings of the reference's example input (G0 = 4 pi AS N a0 / AHO,
+# GDD0 = 3 ADD N a0 / AHO with AS = 100, ADD = 132.7, N = 5000, AHO = 1e-6):
+# a stable ground state (eps_dd = 1.33), unlike the 256^3 workload's G0 = 40,
+# GDD0 = 30 (eps_dd = 3.1), which drive psi towards collapse.
+STABLE = (2 * 332.4918, 2 * 105.33272)
+
+
+def check_imag_256(hip_lib, oracle_lib, coupling, steps, chunk):
     p = dbecpy.make_params(256, 256, 256, 0.1, 0.1, 0.1, 0.0005, 0.5, 1.0, 1.5, 12.8)
     psi0 = dbecpy.gaussian_psi0(p)
     with dbecpy.Solver(hip_lib, p) as g, dbecpy.Solver(oracle_lib, p) as o:
         for s in (g, o):
             s.set_psi(psi0)
             s.calcnorm(rescale=True)
-            s.set_coupling(80.0, 60.0)
-        assert g.step(8) == pytest.approx(o.step(8), rel=TOL)
-        assert rel(g.get_psi(), o.get_psi()) < TOL
-        assert g.calcmuen() == pytest.approx(o.calcmuen(), rel=TOL)
+            s.set_coupling(*coupling)
+        for k in range(steps // chunk):
+            assert g.step(chunk) == pytest.approx(o.step(chunk), rel=TOL)
+            d = rel(g.get_psi(), o.get_psi())
+            mg, mo = g.calcmuen(), o.calcmuen()
+            say(f"  after {chunk * (k + 1)} steps: max|dpsi|/max|psi| {d:.2e}, "
+                f"mu/par {mg[0] / 2:.10f} (|dmu/mu| {abs(mg[0] - mo[0]) / abs(mo[0]):.1e})")
+            assert d < TOL
+            assert mg == pytest.approx(mo, rel=TOL)
+            assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
 
 
-def test_real_256x256x384_short(hip_lib, oracle_lib):
-    """The real-time workload (256x256x384, dx = 0.1, dt = 2e-4, gd quenched to
-    45 from a Gaussian), 3 steps: psi, norm and rms against the oracle."""
-    say("real 256x256x384 x 3")
+def test_imag_256_workload_500_steps(hip_lib, oracle_lib):
+    """The 256^3 workload (dx = 0.1, dt = 5e-4, R = 12.8, G0 = 40, GDD0 = 30,
+    Gaussian psi0), 500 FP64 steps: mu, E, rms and max|d psi| against the
+    oracle every 100 steps at 1e-10."""
+    say("imag 256^3 workload couplings x 500")
+    check_imag_256(hip_lib, oracle_lib, (80.0, 60.0), 500, 100)
+
+
+def test_imag_256_stable_500_steps(hip_lib, oracle_lib):
+    """The same grid on stable couplings (the reference example's G0, GDD0),
+    500 FP64 steps against the oracle at 1e-10."""
+    say("imag 256^3 stable couplings x 500")
+    check_imag_256(hip_lib, oracle_lib, STABLE, 500, 100)
+
+
+def test_real_256x256x384_from_imag_state(hip_lib, oracle_lib):
+    """The real-time workload's grid (256x256x384, dx = 0.1, dt = 2e-4, R =
+    12.8): an imaginary-time FP64 state (200 steps of dt = 5e-4 on stable
+    couplings, on the GPU) propagated for 100 real-time steps with gd
+    quenched by 1.5, GPU against oracle: psi, norm, rms, mu and E (both
+    kinetic forms) at 1e-10."""
+    say("real 256x256x384 x 100 from an imaginary-time state")
+    pi = dbecpy.make_params(256, 256, 384, 0.1, 0.1, 0.1, 0.0005, 0.5, 1.0, 1.5, 12.8)
+    with dbecpy.Solver(hip_lib, pi) as s:
+        s.set_psi(dbecpy.gaussian_psi0(pi))
+        s.calcnorm(rescale=True)
+        s.set_coupling(*STABLE)
+        s.step(200)
+        psi = s.get_psi().astype(np.complex128)
     p = dbecpy.make_params(256, 256, 384, 0.1, 0.1, 0.1, 0.0002, 0.5, 1.0, 1.5, 12.8)
-    psi0 = dbecpy.gaussian_psi0(p).astype(np.complex128)
     with dbecpy.Solver(hip_lib, p, dbecpy.REAL) as g, dbecpy.Solver(oracle_lib, p, dbecpy.REAL) as o:
         for s in (g, o):
-            s.set_psi(psi0)
-            s.set_coupling(2 * 40.0, 2 * 45.0)
-            s.step(3, want_norm=False)
-        assert rel(g.get_psi(), o.get_psi()) < TOL
-        assert g.calcnorm() == pytest.approx(o.calcnorm(), rel=TOL)
-        assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
+            s.set_psi(psi)
+            s.set_coupling(STABLE[0], 1.5 * STABLE[1])
+        del psi
+        for k in range(2):
+            g.step(50, want_norm=False)
+            o.step(50, want_norm=False)
+            d = rel(g.get_psi(), o.get_psi())
+            say(f"  after {50 * (k + 1)} steps: max|dpsi|/max|psi| {d:.2e}")
+            assert d < TOL
+            assert g.calcnorm() == pytest.approx(o.calcnorm(), rel=TOL)
+            assert g.calcrms() == pytest.approx(o.calcrms(), rel=TOL)
+        assert g.calcmuen() == pytest.approx(o.calcmuen(), rel=TOL)
+        assert g.calcmuen_phase() == pytest.approx(o.calcmuen_phase(), rel=TOL)
 
 
 def test_imag_512_fp64_and_mixed(hip_lib, oracle_lib):

@@ -11,6 +11,7 @@
   dipolar request ends (converged to the grid-scale state or flagged
   diverged) instead of running to its step cap.
 """
+import os
 import sys
 
 import numpy as np
@@ -111,3 +112,83 @@ def test_mixed_close_to_fp64_and_collapse_stays_finite(hip_lib):
     h = got[402]
     say(f"  collapsing request: status {h['status']} steps {h['steps']} mu {h['mu']:.6g}")
     assert h["status"] in (ds.CONVERGED, ds.DIVERGED)
+
+
+def test_polish_migration_then_slot_reuse(hip_lib):
+    """Mixed slots of one per bucket: when a ground state moves to its FP64
+    polish, the freed mixed slot takes the next queued request in the same
+    progress call.  The widen that reads the slot is ordered before the new
+    request's initial state is written, so both results equal the same
+    requests served alone, bit for bit."""
+    say("polish migration, then the freed slot is reused")
+    a = stable(500, 48, prec=dbecpy.MIXED)
+    b = stable(501, 48, prec=dbecpy.MIXED, g0=25.0, gd0=6.0, trap=(0.9, 1.1, 1.4))
+    alone = {}
+    for r in (a, b):
+        with ds.Server(hip_lib, max_batch=1, check_every=50, polish_tol=1e-6) as srv:
+            alone.update(srv.run_all([r], timeout=120))
+    with ds.Server(hip_lib, max_batch=1, check_every=50, polish_tol=1e-6) as srv:
+        both = srv.run_all([a, b], timeout=120)
+    for r in (a, b):
+        x, y = alone[r.id], both[r.id]
+        say(f"  id {r.id}: {x['steps']} steps ({x['steps_fp64']} FP64) mu {x['mu']!r} / {y['mu']!r}")
+        assert x["steps_fp64"] > 0
+        for k in ("status", "steps", "steps_fp64", "mu", "en", "norm"):
+            assert x[k] == y[k], k
+        assert x["rms"] == y["rms"]
+
+
+STREAM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "serve_stream.json")
+
+
+def stream_slice():
+    """From the serving stream: the first three ground states and the first
+    two real-time runs with N in {96, 128, 160, 192}, in stream order."""
+    imag, real = [], []
+    for _, r in ds.load_stream(STREAM, prec=dbecpy.MIXED):
+        if r.nx < 96:
+            continue
+        if r.mode == dbecpy.IMAG and len(imag) < 3:
+            imag.append(r)
+        elif r.mode == dbecpy.REAL and len(real) < 2:
+            real.append(r)
+        if len(imag) == 3 and len(real) == 2:
+            return imag + real
+    raise AssertionError("stream slice not found")
+
+
+def test_stream_slice_alone_equals_in_the_mix(hip_lib):
+    """Five requests of the serving stream at its larger grids (96-192), both
+    modes, mixed precision with FP64 polish: served together and each alone,
+    every result is the same bit for bit."""
+    reqs = stream_slice()
+    say("stream slice: " + ", ".join(f"id {r.id} N {r.nx} {'real' if r.mode else 'imag'}" for r in reqs))
+    with ds.Server(hip_lib, max_batch=8, check_every=50) as srv:
+        mix = srv.run_all(reqs, timeout=300)
+    for r in reqs:
+        with ds.Server(hip_lib, max_batch=8, check_every=50) as srv:
+            a = srv.run_all([r], timeout=300)[r.id]
+        b = mix[r.id]
+        say(f"  id {r.id}: status {a['status']} steps {a['steps']} ({a['steps_fp64']} FP64) mu {a['mu']!r}")
+        for k in ("status", "steps", "steps_fp64", "mu", "en", "norm"):
+            assert a[k] == b[k], (r.id, k)
+        assert a["rms"] == b["rms"]
+
+
+def test_stream_slice_fp64_against_oracle(hip_lib, oracle_lib):
+    """A ground state and a real-time run of the stream slice in FP64, 100
+    steps each, served against the same requests solved by the oracle: mu,
+    E and rms at 1e-10."""
+    reqs = stream_slice()
+    picks = [r for r in reqs if r.mode == dbecpy.IMAG][:1] + [r for r in reqs if r.mode == dbecpy.REAL][:1]
+    fp64 = [ds.make_request(r.id, r.nx, r.mode, dbecpy.FP64, r.g / r.par, r.gd / r.par,
+                            (r.vgamma, r.vnu, r.vlambda), r.dt, 100, mutol=1e-300, cutoff=r.cutoff)
+            for r in picks]
+    with ds.Server(hip_lib, max_batch=4, check_every=50) as srv:
+        got = srv.run_all(fp64, timeout=300)
+    for r in fp64:
+        a, b = got[r.id], ds.plain_solve(oracle_lib, r, check_every=50)
+        say(f"  id {r.id} N {r.nx}: mu {a['mu']!r} oracle {b['mu']!r}")
+        assert a["steps"] == b["steps"] == 100
+        assert rel(a["mu"], b["mu"]) < 1e-10 and rel(a["en"], b["en"]) < 1e-10
+        assert np.allclose(a["rms"], b["rms"], rtol=1e-10, atol=0)
