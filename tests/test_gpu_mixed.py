@@ -60,3 +60,50 @@ def test_mixed_then_fp64_polish_converges(hip_lib):
             mud, mup = d.calcmuen()[0], pol.calcmuen()[0]
             say(f"  mu fp64 {mud:.14f} polished {mup:.14f} rel {abs(mup - mud) / abs(mud):.2e}")
             assert abs(mup - mud) / abs(mud) < 1e-8
+
+
+def test_fp64_accumulators_leave_nothing_to_compensate(hip_lib):
+    """The mixed form's reductions accumulate in FP64.  Each term of the
+    Simpson norm is an FP32 value squared (exact in FP64) times an FP64
+    weight, so the device sum differs from a compensated sum of the same
+    terms (math.fsum, exact up to the rounding of each product) by the FP64
+    accumulation error alone.  Measured on a mixed ground state at 128^3:
+    below 1e-13 relative, five orders under the 1e-8 bar and far under the
+    FP32 state error, which is why the sums carry no Kahan compensation."""
+    import math
+
+    say("mixed: FP64 accumulation against a compensated host sum, 128^3")
+    n, h = 128, 0.2
+    p = dbecpy.make_params(n, n, n, h, h, h, 0.005, 0.5, 1.0, 1.5, 10.0)
+    with dbecpy.Solver(hip_lib, p, dbecpy.IMAG, dbecpy.MIXED) as m:
+        m.set_psi(dbecpy.gaussian_psi0(p))
+        m.calcnorm(rescale=True)
+        m.set_coupling(G, GD)
+        m.step(300)
+        psi = m.get_psi()  # FP32 values, held exactly in float64
+        norm_dev = m.calcnorm()
+        rms_dev = m.calcrms()
+
+    def simpson_w(n, h):
+        w = np.zeros(n)
+        for i in range(1, n - 1, 2):
+            w[i - 1] += 1.0
+            w[i] += 4.0
+            w[i + 1] += 1.0
+        if n % 2 == 0:
+            w[n - 1] += 5.0 / 4.0
+            w[n - 2] += 8.0 / 4.0
+            w[n - 3] -= 1.0 / 4.0
+        return w * h / 3.0
+
+    w = simpson_w(n, h)
+    p2 = psi * psi
+    x2 = ((np.arange(n) - n // 2) * h) ** 2
+    rows = np.array([[math.fsum(w * p2[i, j]) for j in range(n)] for i in range(n)])  # z
+    cols = np.array([math.fsum(w * rows[i]) for i in range(n)])  # y
+    norm_host = math.sqrt(math.fsum(w * cols))  # x
+    rx_host = math.sqrt(math.fsum(w * x2 * cols))
+    d_norm = abs(norm_dev - norm_host) / norm_host
+    d_rx = abs(rms_dev[1] - rx_host) / rx_host
+    say(f"  norm device {norm_dev!r} compensated {norm_host!r}: {d_norm:.1e}; <x> {d_rx:.1e}")
+    assert d_norm < 1e-13 and d_rx < 1e-13
