@@ -90,9 +90,39 @@ struct KernelTimer {
 
 }  // namespace dbec
 
+// Device scratch of dbec_density_noalloc: DBEC_NOALLOC_SCRATCH_BYTES, one
+// copy per GPU, held until the process ends.  In host code this name is not a
+// device address; the address of each GPU's copy comes from
+// hipGetSymbolAddress (scratch_setup) and only that address is used.
+static __device__ __align__(256) double scratch[DBEC_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
 using namespace dbec;
 
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;  // device ordinal -> address of that device's scratch
+
+// Once per device, with that device current (dbec_create): look up and keep
+// the address of the device's scratch.  A failure is not fatal here; the
+// allocation-free entry point reports it.
+void scratch_setup(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(dev)) return;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();  // leave no stale error for the next launch check
+        return;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+}
+
+double *scratch_of(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
 
 thread_local std::string g_thread_error;
 
@@ -695,6 +725,7 @@ int dbec_create(const dbec_params *p, dbec_mode mode, dbec_prec prec, void *stre
     if (dev >= ndev) return fail(nullptr, DBEC_EINVAL, "DEVICE out of range");
     e = hipSetDevice(dev);
     if (e != hipSuccess) return fail(nullptr, DBEC_EDEVICE, "hipSetDevice failed");
+    scratch_setup(dev);
     dbec_ctx *c = new (std::nothrow) dbec_ctx();
     if (!c) return fail(nullptr, DBEC_ENOMEM, "out of host memory");
     c->p = *p;
@@ -1019,6 +1050,89 @@ int dbec_density(dbec_ctx *c, int kind, long ox, long oy, long oz, double *out) 
     }
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "density");
+    return DBEC_OK;
+}
+
+int dbec_density_noalloc(dbec_ctx *c, int kind, long ox, long oy, long oz, double *out) {
+    if (!c || !out) return fail(c, DBEC_EINVAL, "null argument");
+    const long cnt = dbec_density_count(c, kind, ox, oy, oz);
+    if (cnt < 0) return fail(c, DBEC_EINVAL, "bad density request");
+    (void)hipSetDevice(c->device);
+    double *d = scratch_of(c->device);
+    if (!d) return fail(c, DBEC_EDEVICE, "dbec_density_noalloc: the device scratch of this GPU has no address");
+    const long Nx = c->Nx, Ny = c->Ny, Nz = c->Nz;
+    const long sx = (Nx + ox - 1) / ox, sy = (Ny + oy - 1) / oy, sz = (Nz + oz - 1) / oz;
+    // doubles of scratch: the weighted sums of the extract, then its sampled values
+    long sums;
+    switch (kind) {
+    case DBEC_DEN_XY: sums = Nx * Ny; break;
+    case DBEC_DEN_X: sums = Nx * Ny + Nx; break;
+    case DBEC_DEN_Y: sums = Nx * Ny + Ny; break;
+    case DBEC_DEN_XZ: sums = Nx * Nz; break;
+    case DBEC_DEN_Z: sums = Nx * Nz + Nz; break;
+    case DBEC_DEN_YZ: sums = Ny * Nz; break;
+    default: sums = 0; break;
+    }
+    const long cap = (long)(DBEC_NOALLOC_SCRATCH_BYTES / sizeof(double));
+    if (sums + cnt > cap)
+        return fail(c, DBEC_EINVAL,
+                    "dbec_density_noalloc: this extract needs " + std::to_string((sums + cnt) * (long)sizeof(double)) +
+                        " bytes of device scratch, more than its " + std::to_string((long)DBEC_NOALLOC_SCRATCH_BYTES) +
+                        " bytes (use dbec_density or coarser output steps)");
+    CHK(materialize_scale(c));
+    const double *wx = c->b.wgt, *wy = c->b.wgt + Nx;
+    double *dst = d + sums;
+    hipStream_t s = c->stream;
+    hipError_t e = hipSuccess;
+    switch (kind) {
+    case DBEC_DEN_XY:
+        e = launch_reduce3(s, c, 2, d);  // [x][y]
+        if (e == hipSuccess) e = launch_pick2(s, d, Ny, ox, oy, sx, sy, dst);
+        break;
+    case DBEC_DEN_X:
+        e = launch_reduce3(s, c, 2, d);
+        if (e == hipSuccess) e = launch_reduce2(s, d, Nx, Ny, 1, wy, d + Nx * Ny);
+        if (e == hipSuccess) e = launch_pick2(s, d + Nx * Ny, 1, ox, 1, sx, 1, dst);
+        break;
+    case DBEC_DEN_Y:
+        e = launch_reduce3(s, c, 2, d);
+        if (e == hipSuccess) e = launch_reduce2(s, d, Nx, Ny, 0, wx, d + Nx * Ny);
+        if (e == hipSuccess) e = launch_pick2(s, d + Nx * Ny, 1, oy, 1, sy, 1, dst);
+        break;
+    case DBEC_DEN_XZ:
+        e = launch_reduce3(s, c, 1, d);  // [x][z]
+        if (e == hipSuccess) e = launch_pick2(s, d, Nz, ox, oz, sx, sz, dst);
+        break;
+    case DBEC_DEN_Z:
+        e = launch_reduce3(s, c, 1, d);
+        if (e == hipSuccess) e = launch_reduce2(s, d, Nx, Nz, 0, wx, d + Nx * Nz);
+        if (e == hipSuccess) e = launch_pick2(s, d + Nx * Nz, 1, oz, 1, sz, 1, dst);
+        break;
+    case DBEC_DEN_YZ:
+        e = launch_reduce3(s, c, 0, d);  // [y][z]
+        if (e == hipSuccess) e = launch_pick2(s, d, Nz, oy, oz, sy, sz, dst);
+        break;
+    default: {
+        long start[3] = {0, 0, 0}, step[3] = {ox, oy, oz}, n[3] = {sx, sy, sz};
+        if (kind == DBEC_CUT_XY0) {
+            start[2] = Nz / 2;
+            step[2] = 1;
+            n[2] = 1;
+        } else if (kind == DBEC_CUT_X0Z) {
+            start[1] = Ny / 2;
+            step[1] = 1;
+            n[1] = 1;
+        } else if (kind == DBEC_CUT_0YZ) {
+            start[0] = Nx / 2;
+            step[0] = 1;
+            n[0] = 1;
+        }
+        e = launch_pick_mod2(s, c, start, step, n, dst);
+    } break;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dst, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "density (no allocation)");
     return DBEC_OK;
 }
 

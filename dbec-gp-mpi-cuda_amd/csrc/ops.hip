@@ -923,4 +923,50 @@ hipError_t launch_mod2(hipStream_t s, const dbec_ctx *c, double *out) {
     return hipGetLastError();
 }
 
+
+// Sampled extracts on the device (the allocation-free density entry point):
+// dst[a][b][k] = |psi|^2 at (x0 + a ox, y0 + b oy, z0 + k oz), the same
+// mod2_of as k_mod2, so the values equal those of the full extract bit for bit.
+template <typename P>
+__global__ void k_pick_mod2(const P *__restrict__ psi, long x0, long y0, long z0, long ox, long oy, long oz, long cy,
+                            long cz, long n, double *__restrict__ dst, long Ny, long Nz) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long a = t / (cy * cz), b = (t / cz) % cy, k = t % cz;
+    dst[t] = mod2_of(psi[((x0 + a * ox) * Ny + y0 + b * oy) * Nz + z0 + k * oz]);
+}
+
+hipError_t launch_pick_mod2(hipStream_t s, const dbec_ctx *c, const long start[3], const long step[3],
+                            const long cnt[3], double *dst) {
+    const long n = cnt[0] * cnt[1] * cnt[2];
+    if (n <= 0) return hipSuccess;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    const bool f32 = c->prec == DBEC_MIXED;
+#define DBEC_PICK_ARGS start[0], start[1], start[2], step[0], step[1], step[2], cnt[1], cnt[2], n, dst, c->Ny, c->Nz
+    if (c->mode == DBEC_IMAG) {
+        if (f32) k_pick_mod2<float><<<g, 256, 0, s>>>(static_cast<const float *>(c->b.psi), DBEC_PICK_ARGS);
+        else k_pick_mod2<double><<<g, 256, 0, s>>>(static_cast<const double *>(c->b.psi), DBEC_PICK_ARGS);
+    } else {
+        if (f32) k_pick_mod2<cpx<float>><<<g, 256, 0, s>>>(static_cast<const cpx<float> *>(c->b.psi), DBEC_PICK_ARGS);
+        else k_pick_mod2<cpx<double>><<<g, 256, 0, s>>>(static_cast<const cpx<double> *>(c->b.psi), DBEC_PICK_ARGS);
+    }
+#undef DBEC_PICK_ARGS
+    return hipGetLastError();
+}
+
+// dst[a][b] = src[(a o0) n1 + b o1], a < c0, b < c1
+__global__ void k_pick2(const double *__restrict__ src, long n1, long o0, long o1, long c1, long n,
+                        double *__restrict__ dst) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    dst[t] = src[(t / c1) * o0 * n1 + (t % c1) * o1];
+}
+
+hipError_t launch_pick2(hipStream_t s, const double *src, long n1, long o0, long o1, long c0, long c1, double *dst) {
+    const long n = c0 * c1;
+    if (n <= 0) return hipSuccess;
+    k_pick2<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, n1, o0, o1, c1, n, dst);
+    return hipGetLastError();
+}
+
 }  // namespace dbec

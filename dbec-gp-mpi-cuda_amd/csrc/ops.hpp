@@ -61,5 +61,10 @@ hipError_t launch_finalize(hipStream_t s, const dbec_ctx *c, int mode, const dou
 hipError_t launch_reduce3(hipStream_t s, const dbec_ctx *c, int axis, double *out);
 hipError_t launch_reduce2(hipStream_t s, const double *in, long n0, long n1, int axis, const double *w, double *out);
 hipError_t launch_mod2(hipStream_t s, const dbec_ctx *c, double *out);
+// dst[a][b][k] = |psi|^2 at (start + (a, b, k) * step), (a, b, k) < cnt
+hipError_t launch_pick_mod2(hipStream_t s, const dbec_ctx *c, const long start[3], const long step[3],
+                            const long cnt[3], double *dst);
+// dst[a][b] = src[(a o0) n1 + b o1], a < c0, b < c1
+hipError_t launch_pick2(hipStream_t s, const double *src, long n1, long o0, long o1, long c0, long c1, double *dst);
 
 }  // namespace dbec

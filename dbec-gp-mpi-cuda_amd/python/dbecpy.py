@@ -28,7 +28,7 @@ SYMBOLS = [
     "dbec_set_coupling", "dbec_get_coupling", "dbec_calcpsidd2", "dbec_get_psidd2",
     "dbec_calcnu", "dbec_calclux", "dbec_calcluy", "dbec_calcluz", "dbec_calcnorm",
     "dbec_calcmuen", "dbec_calcmuen_phase", "dbec_calcrms", "dbec_psi_center", "dbec_step", "dbec_density_count",
-    "dbec_density", "dbec_get_table", "dbec_synchronize", "dbec_profile_step", "dbec_tuning",
+    "dbec_density", "dbec_density_noalloc", "dbec_get_table", "dbec_synchronize", "dbec_profile_step", "dbec_tuning",
 ]
 
 
@@ -82,6 +82,7 @@ def load(path=HIP_LIB):
         "dbec_step": ([P, L, ctypes.c_double, ctypes.c_double, D], I),
         "dbec_density_count": ([P, I, L, L, L], L),
         "dbec_density": ([P, I, L, L, L, D], I),
+        "dbec_density_noalloc": ([P, I, L, L, L, D], I),
         "dbec_get_table": ([P, I, I, D], I),
         "dbec_synchronize": ([P], I),
         "dbec_profile_step": ([P, I, ctypes.POINTER(KernelStat), I, ctypes.POINTER(I)], I),
@@ -242,6 +243,16 @@ class Solver:
             raise DbecError(DBEC_EINVAL, "bad density request")
         a = np.empty(n, dtype=np.float64)
         self._chk(self.lib.dbec_density(self.h, kind, ox, oy, oz, _dp(a)))
+        return a
+
+    def density_noalloc(self, kind, ox=1, oy=1, oz=1):
+        """dbec_density_noalloc: the same extract through the library's fixed
+        device scratch (no device allocation)."""
+        n = self.lib.dbec_density_count(self.h, kind, ox, oy, oz)
+        if n < 0:
+            raise DbecError(DBEC_EINVAL, "bad density request")
+        a = np.empty(n, dtype=np.float64)
+        self._chk(self.lib.dbec_density_noalloc(self.h, kind, ox, oy, oz, _dp(a)))
         return a
 
     def table(self, which, axis=0):

@@ -4,8 +4,9 @@
  *
  * One opaque context per solve.  The context owns every device buffer; the
  * caller owns host memory.  All entry points return an int status (0 = OK,
- * see DBEC_E*), never exit the process, and keep no global state, so many
- * contexts coexist in one process.  Operators are stream ordered; entry
+ * see DBEC_E*), never exit the process, and keep no global state (apart from
+ * the fixed device scratch of dbec_density_noalloc), so many contexts coexist
+ * in one process.  Operators are stream ordered; entry
  * points that return scalars to the host synchronise the context's stream.
  *
  * Host arrays are row-major [x][y][z] (z fastest).  Imaginary-time contexts
@@ -126,6 +127,20 @@ int dbec_step(dbec_ctx *ctx, long nsteps, double g_stp, double gd_stp, double *n
 #define DBEC_DEN_XYZ  9   /* |psi|^2, [x][y][z] sampled           (outdenxyz) */
 long dbec_density_count(const dbec_ctx *ctx, int kind, long outstpx, long outstpy, long outstpz);
 int dbec_density(dbec_ctx *ctx, int kind, long outstpx, long outstpy, long outstpz, double *out);
+
+/* dbec_density without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  The same extracts, equal to
+ * dbec_density's bit for bit; the weighted sums and the sampled values live
+ * in a fixed device array of DBEC_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, instead of a buffer allocated per call.  An extract whose
+ * sums and samples need more fails with DBEC_EINVAL and a message: every 1-D
+ * and 2-D kind fits at every supported grid, the sampled 3-D extracts up to
+ * 1048576 values.  All calls on one GPU share its array and the call ends
+ * with a synchronisation of the context's stream: calls that overlap on one
+ * GPU (from several threads or streams) race, so serialise them. */
+#define DBEC_NOALLOC_SCRATCH_BYTES (8L << 20)
+int dbec_density_noalloc(dbec_ctx *ctx, int kind, long outstpx, long outstpy, long outstpz, double *out);
 
 /* Set-up tables, for tests (host copies).  POT: [x][y][z]; POTDD: [kx][ky][kz]
  * for kz <= Nz/2; CALPHA/CGAMMA: axis 0,1,2, length N-1 (real: 1 double per

@@ -886,6 +886,12 @@ int dbec_get_table(dbec_ctx *c, int which, int axis, double *out) {
     }
 }
 
+/* The allocation-free variant computes the same extracts; the oracle holds
+ * no device memory, so it is dbec_density itself. */
+int dbec_density_noalloc(dbec_ctx *c, int kind, long ox, long oy, long oz, double *out) {
+    return dbec_density(c, kind, ox, oy, oz, out);
+}
+
 int dbec_synchronize(dbec_ctx *c) { return c ? DBEC_OK : DBEC_EINVAL; }
 
 int dbec_tuning(const dbec_ctx *c, char *buf, long len) {
