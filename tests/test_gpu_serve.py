@@ -170,9 +170,12 @@ def test_stream_slice_alone_equals_in_the_mix(hip_lib):
             a = srv.run_all([r], timeout=300)[r.id]
         b = mix[r.id]
         say(f"  id {r.id}: status {a['status']} steps {a['steps']} ({a['steps_fp64']} FP64) mu {a['mu']!r}")
-        for k in ("status", "steps", "steps_fp64", "mu", "en", "norm"):
+        for k in ("status", "steps", "steps_fp64"):
             assert a[k] == b[k], (r.id, k)
-        assert a["rms"] == b["rms"]
+        # bit for bit; a diverged (collapsing) solve reports NaN in both runs
+        for k in ("mu", "en", "norm"):
+            assert np.array_equal(np.float64(a[k]), np.float64(b[k]), equal_nan=True), (r.id, k)
+        assert np.array_equal(np.array(a["rms"]), np.array(b["rms"]), equal_nan=True)
 
 
 def test_stream_slice_fp64_against_oracle(hip_lib, oracle_lib):
