@@ -128,7 +128,7 @@ def percentiles(xs):
             "p99": float(np.percentile(a, 99))}
 
 
-def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeout):
+def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeout, capacity_only=False):
     """The serving stream (first `count` requests, Poisson arrivals at `rate`
     per second, mixed precision) replayed in real time against the request
     loop, and its first `baseline_count` requests as back-to-back plain FP64
@@ -146,6 +146,9 @@ def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeou
         srv.run_all([r for _, r in stream], timeout=timeout)
         cap_wall = time.perf_counter() - t0
     log(f"serving capacity: {count} requests at once in {cap_wall:.2f} s = {count / cap_wall:.2f} solves/s")
+    if capacity_only:
+        return {"requests": count, "precision": "mixed", "max_batch": max_batch, "check_every": check_every,
+                "capacity_solves_per_s": count / cap_wall, "capacity_wall_s": cap_wall}
     with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every) as srv:
         t0 = time.perf_counter()
         res = ds.serve_stream(srv, stream, timeout=timeout)
@@ -304,6 +307,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-batch", type=int, default=16)
     ap.add_argument("--serve-check", type=int, default=50)
+    ap.add_argument("--serve-capacity-only", action="store_true",
+                    help="run only the serving capacity measurement (the first --serve-requests requests at once)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -318,6 +323,12 @@ def main():
 
     import dbecpy
 
+    if args.serve_capacity_only:
+        lib = dbecpy.load(dbecpy.HIP_LIB)
+        out = serve_bench(lib, args.serve_requests, 0, args.serve_rate, args.serve_batch, args.serve_check,
+                          timeout=600.0, capacity_only=True)
+        print(json.dumps({"serving_capacity": out}), flush=True)
+        return
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb << 20)
         log(f"copy calibration: {args.copy_calibration_mb} MiB device-to-device")
@@ -334,16 +345,6 @@ def main():
     s.set_psi(dbecpy.gaussian_psi0(p))
     s.calcnorm(rescale=(m == dbecpy.IMAG))
     s.set_coupling(80.0, 60.0)
-    # per-kernel HIP-event timing of --profile-reps steps; it runs before the
-    # timed region, so the device is at its working clocks when that starts
-    stats = s.profile(args.profile_reps)
-    log("per-kernel: " + ", ".join(f"{k['name']} {k['avg_ms'] * 1e3:.1f}us" for k in stats))
-    dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
-    achieved = dom["alg_bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
-    step_kernel_ms = sum(k["avg_ms"] * k["launches"] for k in stats)
-    # kernels that run once per dbec_step call (the first forward z pass, the
-    # last z sweep, the final norm and rescale) count once per timed call
-    step_bytes = sum(k["alg_bytes"] * (k["launches"] if k["launches"] >= 1 else 1.0 / args.steps) for k in stats)
     s.step(args.warmup, want_norm=False)
     s.synchronize()
     log(f"warmup {args.warmup} steps done")
@@ -356,7 +357,21 @@ def main():
     ms_step = 1e3 * t / args.steps
     norm = s.calcnorm()
     mu, en = s.calcmuen()
+    # per-kernel HIP-event timing right after the timed region (the device at
+    # its working clocks): every launch of a few steps repeated
+    # --profile-reps times back to back on the solver's stream, so each
+    # figure is the pipelined time of one launch.  The repeats alter psi, so
+    # this comes after the observables.
+    stats = s.profile(args.profile_reps)
     s.close()
+    log("per-kernel: " + ", ".join(f"{k['name']} {k['avg_ms'] * 1e3:.1f}us" for k in stats))
+    dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
+    achieved = dom["alg_bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    step_kernel_ms = sum(k["avg_ms"] * k["launches"] for k in stats)
+    # kernels that run once per dbec_step call (the first forward z pass, the
+    # last z sweep, the final norm and rescale; launches 0) count once per
+    # timed call
+    step_bytes = sum(k["alg_bytes"] * (k["launches"] if k["launches"] >= 1 else 1.0 / args.steps) for k in stats)
 
     balg = b_alg_per_point(mode, prec, n[2])
     line = {

@@ -551,9 +551,10 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
 // rows x < Nx-1, y < Ny-1: within such a row the face copy of
 // calcpsidd2_kernel2b (imag3d.cu:1618-1624) takes phi[.][.][Nz-1] from
 // phi[.][.][0] of the same row, so the row's potential never leaves the chip.
-// psi of 8 bytes or less per element is fetched into registers before the
-// transform, so its latency overlaps the spectrum load; complex FP64 psi is
-// fetched after it in batches of 8.  The per-row and per-z parts of the trap
+// The tile's psi is fetched into registers before the transform when it is at
+// most 256 bytes per thread (every plan with T = N / 8 or N / 4), so its
+// latency overlaps the spectrum load and the passes; otherwise after it in
+// batches of 8.  The per-row and per-z parts of the trap
 // potential (trap_xy, trap_z) are staged in LDS.  With ZREG the spectrum
 // pairs go straight into the registers of the first transform pass
 // (load_hermitian_reg) instead of through LDS; with SWZ the transform buffer
@@ -565,7 +566,9 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     constexpr int L = FftTile<N, LM>::L, WG = FftTile<N, LM>::WG;
     using Pl = FftPlan<N>;
     constexpr int Q = 2 * L * N / WG;  // psi elements per thread (interior rows)
-    constexpr bool PREFETCH = sizeof(P) <= 8;
+    // the tile's psi (at most 256 bytes per thread) is fetched before the
+    // transform, so its latency overlaps the spectrum load and the passes
+    constexpr bool PREFETCH = Q * sizeof(P) <= 256;
     constexpr int C = PREFETCH ? Q : (Q < 8 ? Q : 8);  // psi batch
     static_assert((2 * L * N) % WG == 0 && Q % C == 0, "zinv_nu tile");
     __shared__ cpx<R> buf[N * lds_slots_per_row<L, SWZ>()];

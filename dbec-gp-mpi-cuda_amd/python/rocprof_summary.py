@@ -4,6 +4,7 @@
     rocprof_summary.py stats  <run_kernel_stats.csv>                 [--title T] [--cmd C]
     rocprof_summary.py traffic <fetch counter_collection.csv> <write counter_collection.csv>
                                --workload W [--copy-mib M] [--cmd C]
+    rocprof_summary.py busy   <run_kernel_trace.csv>                 [--title T] [--cmd C] [--top K]
 
 `stats` prints a Markdown table of per-kernel calls and durations (from
 `rocprofv3 --kernel-trace --stats --output-format csv`).  A kernel that ran
@@ -16,7 +17,8 @@ dispatch; each is scaled by what it reads for the runtime's device-to-device
 copy kernel (`__amd_rocclr_copyBuffer`) of a known size M MiB that the
 profiled program ran first (bench.py --copy-calibration-mb M): a copy of M MiB
 fetches and writes M MiB, so factor = M MiB / sum of the counter over the copy
-dispatches.
+dispatches.  `busy` reports how much of a kernel trace's span the device had a
+kernel running, and how much kernels of several streams overlapped.
 """
 import argparse
 import csv
@@ -92,6 +94,56 @@ def cmd_stats(args):
         print(f"| {n} | {calls} | {tot / calls / 1e3:.1f} | {100 * tot / total:.1f}% | `{full[:110]}` |")
 
 
+def first_targ(full):
+    """The first template argument of a dbec kernel (its transform length or
+    storage type), or ''."""
+    m = re.search(r"dbec::(k_\w+)", full)
+    if not m or full[m.end():m.end() + 1] != "<":
+        return ""
+    targs = template_args(full, m.end())
+    return targs[0] if targs else ""
+
+
+def cmd_busy(args):
+    """Device occupancy of a kernel trace (rocprofv3 --kernel-trace
+    run_kernel_trace.csv): the span from the first kernel's start to the last
+    one's end, the union of the kernels' intervals (time with at least one
+    kernel running), their summed durations (more than the union when
+    kernels of several streams overlap), and the summed time per kernel and
+    first template argument."""
+    iv, per = [], defaultdict(lambda: [0, 0.0])
+    for r in csv.DictReader(open(args.csv)):
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        iv.append((a, b))
+        name = r["Kernel_Name"]
+        key = (short_name(name) or name[:40], first_targ(name))
+        per[key][0] += 1
+        per[key][1] += b - a
+    iv.sort()
+    span = iv[-1][1] - iv[0][0]
+    union, cur_a, cur_b = 0, iv[0][0], iv[0][1]
+    for a, b in iv[1:]:
+        if a > cur_b:
+            union += cur_b - cur_a
+            cur_a, cur_b = a, b
+        else:
+            cur_b = max(cur_b, b)
+    union += cur_b - cur_a
+    summed = sum(b - a for a, b in iv)
+    print(f"# {args.title}\n")
+    if args.cmd:
+        print(f"Command: `{args.cmd}`\n")
+    print(f"* kernels: {len(iv)}")
+    print(f"* span, first kernel start to last kernel end: {span / 1e9:.3f} s")
+    print(f"* time with at least one kernel running: {union / 1e9:.3f} s ({100 * union / span:.1f} % of the span)")
+    print(f"* summed kernel durations: {summed / 1e9:.3f} s ({summed / union:.2f}x the busy time: kernels of "
+          f"several streams overlapping)\n")
+    print("| kernel | template arg | calls | total s | share of summed |")
+    print("|---|---|---:|---:|---:|")
+    for (n, t), (c, d) in sorted(per.items(), key=lambda kv: -kv[1][1])[: args.top]:
+        print(f"| {n} | {t} | {c} | {d / 1e9:.3f} | {100 * d / summed:.1f}% |")
+
+
 def counter_avg(path, counter):
     per = defaultdict(list)
     for r in csv.DictReader(open(path)):
@@ -153,6 +205,11 @@ def main():
     a.add_argument("csv")
     a.add_argument("--title", default="rocprofv3 kernel statistics")
     a.add_argument("--cmd", default="")
+    c = sub.add_parser("busy")
+    c.add_argument("csv")
+    c.add_argument("--title", default="Device occupancy of a kernel trace")
+    c.add_argument("--cmd", default="")
+    c.add_argument("--top", type=int, default=25)
     b = sub.add_parser("traffic")
     b.add_argument("fetch_csv")
     b.add_argument("write_csv")
@@ -160,7 +217,7 @@ def main():
     b.add_argument("--copy-mib", type=float, default=512)
     b.add_argument("--cmd", default="")
     args = ap.parse_args()
-    cmd_stats(args) if args.what == "stats" else cmd_traffic(args)
+    {"stats": cmd_stats, "traffic": cmd_traffic, "busy": cmd_busy}[args.what](args)
 
 
 if __name__ == "__main__":

@@ -161,10 +161,14 @@ int dbec_tuning(const dbec_ctx *ctx, char *buf, long len);
 /* Wait for all work queued on the context's stream. */
 int dbec_synchronize(dbec_ctx *ctx);
 
-/* Per-kernel timing of one composite step, measured with HIP events on the
- * context's stream over `reps` steps.  Fills up to `max` records; *n gets the
- * number of kernels in one step.  No counterpart in the reference (its only
- * timers bracket whole phase loops, imag3d.cu:314/326). */
+/* Per-kernel timing of the composite step, measured with HIP events on the
+ * context's stream: calls of two and of three steps run, and each kernel
+ * launch of them is repeated `reps` times back to back between two events,
+ * so avg_ms is the pipelined time of one launch; `launches` is per step of a
+ * long run (0 for kernels that run once per dbec_step call).  The repeats change psi: the state after the call is
+ * not a solver state (profile after reading the results you need).  Fills up
+ * to `max` records; *n gets the number of kernels.  No counterpart in the
+ * reference (its only timers bracket whole phase loops, imag3d.cu:314/326). */
 typedef struct dbec_kernel_stat {
     char name[48];
     double avg_ms;         /* mean duration of one launch */

@@ -79,3 +79,20 @@ def test_traffic_picks_the_dominant_variant_per_counter_pass(tmp_path):
     assert xc["fetch_bytes"] == 2.0 * 1024 * 50000 and xc["write_bytes"] == 1024 * 70000
     assert xc["fetch_variant"] == XC2 and xc["write_variant"] == XC1
     assert "different variants" in r.stderr
+
+
+def test_busy_union_and_overlap(tmp_path):
+    """Two streams: kernels [0, 10) and [5, 15) overlap, then a gap, then
+    [20, 30): span 30, union 25, summed 30 (1.2x the busy time)."""
+    p = tmp_path / "trace.csv"
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        for name, a, b in ((XC1, 0, 10_000_000), (YF, 5_000_000, 15_000_000), (XC2, 20_000_000, 30_000_000)):
+            w.writerow({"Kernel_Name": name, "Start_Timestamp": a, "End_Timestamp": b})
+    out = run(["busy", str(p)])
+    assert "span, first kernel start to last kernel end: 0.030 s" in out
+    assert "at least one kernel running: 0.025 s (83.3 % of the span)" in out
+    assert "summed kernel durations: 0.030 s (1.20x" in out
+    rows = [l for l in out.splitlines() if l.startswith("| xconv")]
+    assert rows and "| 256 | 2 | 0.020 |" in rows[0]
