@@ -450,13 +450,16 @@ def main():
             line["serving"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # before the oracle runs: its OpenMP runtime binds the calling thread
+        # to one CPU, which would then be all that the affinity query sees
+        host = host_description(threads)
         log(f"cpu baseline: oracle, {args.cpu_steps} steps at {n}, {threads} threads")
         v, tc = cpu_baseline(n, h, dt, cutoff, mode, args.cpu_steps, threads)
         line["cpu_baseline"] = {
             "value": v, "unit": "grid-pt-steps/s", "cores": threads, "kind": "port",
             "sample": f"{args.cpu_steps} FP64 split steps of the same {n[0]}x{n[1]}x{n[2]} {mode}-time workload "
                       f"({tc:.1f} s), OpenMP oracle",
-            "host": host_description(threads),
+            "host": host,
         }
         if args.cpu_cfg0:
             # the 64^3 CPU configuration in full: dx = 0.25, dt = 0.002, 2000
