@@ -128,7 +128,8 @@ def percentiles(xs):
             "p99": float(np.percentile(a, 99))}
 
 
-def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeout, capacity_only=False):
+def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeout, capacity_only=False,
+                polish_tol=None):
     """The serving stream (first `count` requests, Poisson arrivals at `rate`
     per second, mixed precision) replayed in real time against the request
     loop, and its first `baseline_count` requests as back-to-back plain FP64
@@ -141,15 +142,18 @@ def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeou
     if rate != 20.0:  # the fixture's arrivals are at 20 requests/s
         stream = [(t * 20.0 / rate, r) for t, r in stream]
     # capacity: the same requests all submitted at once
-    with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every) as srv:
+    if polish_tol is None:
+        polish_tol = ds.DEFAULT_POLISH_TOL
+    kw = {"polish_tol": polish_tol}
+    with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every, **kw) as srv:
         t0 = time.perf_counter()
         srv.run_all([r for _, r in stream], timeout=timeout)
         cap_wall = time.perf_counter() - t0
     log(f"serving capacity: {count} requests at once in {cap_wall:.2f} s = {count / cap_wall:.2f} solves/s")
     if capacity_only:
         return {"requests": count, "precision": "mixed", "max_batch": max_batch, "check_every": check_every,
-                "capacity_solves_per_s": count / cap_wall, "capacity_wall_s": cap_wall}
-    with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every) as srv:
+                "polish_tol": polish_tol, "capacity_solves_per_s": count / cap_wall, "capacity_wall_s": cap_wall}
+    with ds.Server(lib, device=0, max_batch=max_batch, check_every=check_every, **kw) as srv:
         t0 = time.perf_counter()
         res = ds.serve_stream(srv, stream, timeout=timeout)
         wall = time.perf_counter() - t0
@@ -159,7 +163,7 @@ def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeou
         status[r["status"]] = status.get(r["status"], 0) + 1
     out = {
         "requests": count, "rate_per_s": rate, "precision": "mixed", "max_batch": max_batch,
-        "check_every": check_every, "capacity_solves_per_s": count / cap_wall,
+        "check_every": check_every, "polish_tol": polish_tol, "capacity_solves_per_s": count / cap_wall,
         "wall_s": wall, "solves_per_s": count / wall,
         "latency_s": percentiles(lat), "mean_steps": float(np.mean([r["steps"] for r in res])),
         "status_counts": {["converged", "steps_done", "not_converged", "diverged"][k]: v for k, v in status.items()},
@@ -307,6 +311,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-batch", type=int, default=16)
     ap.add_argument("--serve-check", type=int, default=50)
+    ap.add_argument("--serve-polish-tol", type=float, default=None,
+                    help="request loop polish_tol (default: the binding's)")
     ap.add_argument("--serve-capacity-only", action="store_true",
                     help="run only the serving capacity measurement (the first --serve-requests requests at once)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -326,7 +332,7 @@ def main():
     if args.serve_capacity_only:
         lib = dbecpy.load(dbecpy.HIP_LIB)
         out = serve_bench(lib, args.serve_requests, 0, args.serve_rate, args.serve_batch, args.serve_check,
-                          timeout=600.0, capacity_only=True)
+                          timeout=600.0, capacity_only=True, polish_tol=args.serve_polish_tol)
         print(json.dumps({"serving_capacity": out}), flush=True)
         return
     if args.copy_calibration_mb:
@@ -444,7 +450,8 @@ def main():
     if args.serve_requests and mode == "imag" and prec == "fp64":
         try:
             line["serving"] = serve_bench(lib, args.serve_requests, args.serve_baseline, args.serve_rate,
-                                          args.serve_batch, args.serve_check, timeout=600.0)
+                                          args.serve_batch, args.serve_check, timeout=600.0,
+                                          polish_tol=args.serve_polish_tol)
         except Exception as e:  # report, keep the throughput line
             log(f"serving benchmark failed: {e!r}")
             line["serving"] = {"error": repr(e)}

@@ -63,11 +63,19 @@ double now_since(std::chrono::steady_clock::time_point t0) {
 
 size_t round_up(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Chunks of a real-time bucket launched back to back before the host looks
+// again (admission into the bucket waits at most this many chunks).
+constexpr long kChainMax = 8;
+// |dmu/mu| below which a mixed ground state whose mu changes stop falling is
+// at FP32's rounding floor (mu of an FP32 wave function carries about 1e-7).
+constexpr double kPolishFloor = 1e-6;
+
 struct Job {
     dbec_request r;
     double t_submit = 0., t_start = 0.;
     long steps = 0, steps_fp64 = 0;
     double mu_prev = 0.;
+    double dmu_prev = -1.;  // |dmu/mu| of the previous check (< 0: none yet)
     bool have_prev = false;
     bool polishing = false;  // moved to an FP64 slot
     bool to_polish = false;  // mixed phase done, waiting for an FP64 slot
@@ -381,6 +389,16 @@ struct dbec_server {
         }
         const int na = (int)b->launched.size();
         if (na == 0) return DBEC_OK;
+        // Real-time solves need no decision between chunks: while every
+        // active one has at least R more chunks to run, R chunks go out
+        // back to back (at most kChainMax), so the device does not wait for
+        // the host between them.  Ground states are checked every chunk.
+        int R = 1;
+        if (!imag && K == cfg.check_every) {
+            long rmin = kChainMax;
+            for (int s : b->launched) rmin = std::min(rmin, (b->job[s]->r.max_steps - b->job[s]->steps) / K);
+            R = (int)std::max(1L, rmin);
+        }
         const bool fp64 = t->prec == DBEC_FP64;
         for (int i = 0; i < na; i++) b->h_slot[i] = b->launched[i];
         hipStream_t st = t->stream;
@@ -433,13 +451,14 @@ struct dbec_server {
                     return fail(std::string("chunk graph: ") + hipGetErrorString(e));
                 }
             }
-            e = hipGraphLaunch(b->graphs[na], st);
+            for (int c = 0; c < R && e == hipSuccess; c++) e = hipGraphLaunch(b->graphs[na], st);
             if (e != hipSuccess) return fail(std::string("chunk graph launch: ") + hipGetErrorString(e));
         } else {
-            for (long k = 0; k < K; k++) {
-                int rc = core_step(t, k ? vf : v, fuse && k > 0, fuse && k + 1 < K);
-                if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
-            }
+            for (int c = 0; c < R; c++)
+                for (long k = 0; k < K; k++) {
+                    int rc = core_step(t, k ? vf : v, fuse && k > 0, fuse && k + 1 < K);
+                    if (rc != DBEC_OK) return fail(std::string("step: ") + t->err);
+                }
             if (imag) {
                 int rc = core_norm(t, v);
                 if (rc != DBEC_OK) return fail(std::string("norm: ") + t->err);
@@ -447,8 +466,8 @@ struct dbec_server {
         }
         for (int s : b->launched) {
             Job &j = *b->job[s];
-            j.steps += K;
-            if (fp64 && j.polishing) j.steps_fp64 += K;
+            j.steps += K * R;
+            if (fp64 && j.polishing) j.steps_fp64 += K * R;
             int what = 0;
             if (imag) what = OBS_MUEN | OBS_RMS;
             else if (j.steps >= j.r.max_steps) what = OBS_MUEN | OBS_RMS;
@@ -476,10 +495,17 @@ struct dbec_server {
                 const double mu = sc[SC_MU];
                 const double dmu = std::fabs(mu - j.mu_prev);
                 const bool polish = b->tab->prec == DBEC_MIXED && cfg.polish_tol > j.r.mutol;
+                const double rel = dmu / std::fabs(mu);
+                // FP32 resolves mu to about 1e-7: once |dmu/mu| is below
+                // kPolishFloor and no longer falls between checks, the mixed
+                // phase is at its rounding floor and moves on even if
+                // polish_tol is below that floor
+                const bool floor = j.dmu_prev >= 0. && rel < kPolishFloor && rel >= j.dmu_prev;
                 if (!std::isfinite(mu) || sc[SC_CAPPED] != 0.) status = DBEC_SOLVE_DIVERGED;
-                else if (polish && j.have_prev && dmu <= cfg.polish_tol * std::fabs(mu)) j.to_polish = true;
+                else if (polish && j.have_prev && (rel <= cfg.polish_tol || floor)) j.to_polish = true;
                 else if (j.have_prev && dmu <= j.r.mutol * std::fabs(mu)) status = DBEC_SOLVE_CONVERGED;
                 else if (j.steps >= j.r.max_steps) status = DBEC_SOLVE_NOT_CONVERGED;
+                j.dmu_prev = j.have_prev ? rel : -1.;
                 j.mu_prev = mu;
                 j.have_prev = true;
             } else if (j.steps >= j.r.max_steps) {

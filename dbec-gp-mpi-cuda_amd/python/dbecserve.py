@@ -87,8 +87,13 @@ def make_request(rid, n, mode, prec, g0, gd0, trap, dt, max_steps, mutol=1e-8, c
                    g=par * g0, gd=par * gd0, max_steps=max_steps, mutol=mutol)
 
 
+# A mixed ground state moves to FP64 once |dmu/mu| <= this between checks (or
+# once it stops falling below 1e-6, FP32's rounding floor).
+DEFAULT_POLISH_TOL = 1e-6
+
+
 class Server:
-    def __init__(self, lib, device=0, mem_budget=0.0, max_batch=16, check_every=50, polish_tol=1e-6):
+    def __init__(self, lib, device=0, mem_budget=0.0, max_batch=16, check_every=50, polish_tol=DEFAULT_POLISH_TOL):
         self.lib = declare(lib)
         self.h = ctypes.c_void_p()
         cfg = Config(device=device, mem_budget=mem_budget, max_batch=max_batch, check_every=check_every,

@@ -16,7 +16,8 @@
  * spectrum.  Each bucket advances all of its active solves with one launch per
  * kernel (the step kernels take a list of slots), on its own stream, in chunks
  * of `check_every` steps; between chunks the host admits waiting requests into
- * free slots and retires finished ones.  Buckets are created while the device
+ * free slots and retires finished ones (a bucket of real-time solves runs up
+ * to 8 chunks back to back when every solve in it has that many left).  Buckets are created while the device
  * memory they need fits `mem_budget`, and idle buckets are released when a new
  * one does not fit.  A solve's result does not depend on which other solves
  * shared its launches.
@@ -41,7 +42,9 @@ typedef struct {
     long check_every;    /* steps per chunk: mu check, admission and retirement points */
     double polish_tol;   /* > 0: a DBEC_MIXED ground state that reaches |dmu/mu| <= polish_tol moves
                             to an FP64 slot of the same grid and converges there to its mutol
-                            (FP32 cannot resolve mu to 1e-8); <= 0: mixed solves run to mutol */
+                            (FP32 cannot resolve mu to 1e-8); it also moves once |dmu/mu| is below
+                            1e-6 and stops falling between checks (FP32's rounding floor);
+                            <= 0: mixed solves run to mutol */
 } dbec_server_config;
 
 typedef struct {
