@@ -459,6 +459,7 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         # before the oracle runs: its OpenMP runtime binds the calling thread
         # to one CPU, which would then be all that the affinity query sees
+        os.environ.setdefault("OMP_PROC_BIND", "close")
         host = host_description(threads)
         log(f"cpu baseline: oracle, {args.cpu_steps} steps at {n}, {threads} threads")
         v, tc = cpu_baseline(n, h, dt, cutoff, mode, args.cpu_steps, threads)
