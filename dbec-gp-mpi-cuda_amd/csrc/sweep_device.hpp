@@ -80,10 +80,47 @@ __device__ __forceinline__ cpx<float> coef_round(cpx<float>, cpx<double> v) { re
 template <typename S> __device__ __forceinline__ S to_a(S v) { return v; }
 template <typename S> __device__ __forceinline__ S from_a(S v) { return v; }
 
+// Cross-lane moves of a double as two 32-bit halves.  DPP (data-parallel
+// primitives) reads another lane of the same 16-lane row inside the VALU
+// instruction; ds_swizzle permutes within 32 lanes through the LDS crossbar
+// without touching LDS memory.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int PATTERN>
+__device__ __forceinline__ double swizzle_d(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)b, PATTERN);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)(b >> 32), PATTERN);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes active), returned in every
+// lane: DPP quad permutations (lane ^ 1, lane ^ 2), DPP row half-mirror and
+// row mirror (the 8- and 16-lane sums), a ds_swizzle bit-mask step (lane ^
+// 16: the 32-lane sums) and the two 32-lane halves read as scalars.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    constexpr int kQuadXor1 = 0xB1;       // quad_perm [1, 0, 3, 2]
+    constexpr int kQuadXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
+    constexpr int kRowHalfMirror = 0x141;  // lane i <-> 7 - i within each 8 lanes
+    constexpr int kRowMirror = 0x140;      // lane i <-> 15 - i within each row of 16
+    constexpr int kSwizzleXor16 = 0x1F | (0x10 << 10);  // bit-mask mode: and 0x1f, or 0, xor 0x10
+    v += dpp_d<kQuadXor1>(v);
+    v += dpp_d<kQuadXor2>(v);
+    v += dpp_d<kRowHalfMirror>(v);
+    v += dpp_d<kRowMirror>(v);
+    v += swizzle_d<kSwizzleXor16>(v);
+    return readlane_d(v, 0) + readlane_d(v, 32);
 }
 
 // Device coefficient tables of one axis: calpha and cgamma (n - 1 each), and
