@@ -108,7 +108,9 @@ __device__ __forceinline__ void sincos_phase(double t, double *sn, double *cs) {
     pc = fma(pc, -r2, 0.5);                       // 1/2!
     const double c = fma(-r2, pc, 1.0);
     // t = r + q pi/2: sin t = s, c, -s, -c and cos t = c, -s, -c, s for q = 0..3
-    const int q = (int)(long long)k & 3;
+    // (k clamped so that the conversion is defined for any t, NaN included;
+    // s and c are NaN then anyway)
+    const int q = (int)(long long)fmin(fmax(k, -4.0e15), 4.0e15) & 3;
     const double a = (q & 1) ? c : s, bq = (q & 1) ? s : c;
     *sn = (q & 2) ? -a : a;
     *cs = ((q + 1) & 2) ? -bq : bq;
@@ -132,7 +134,7 @@ __device__ __forceinline__ void sincos_phase(float t, float *sn, float *cs) {
     pc = fmaf(pc, -r2, 1.0f / 24.0f);             // 1/4!
     pc = fmaf(pc, -r2, 0.5f);                     // 1/2!
     const float c = fmaf(-r2, pc, 1.0f);
-    const int q = (int)k & 3;
+    const int q = (int)fminf(fmaxf(k, -1.0e9f), 1.0e9f) & 3;
     const float a = (q & 1) ? c : s, bq = (q & 1) ? s : c;
     *sn = (q & 2) ? -a : a;
     *cs = ((q + 1) & 2) ? -bq : bq;
