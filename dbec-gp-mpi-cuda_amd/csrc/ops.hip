@@ -329,8 +329,10 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
 }
 
 // Segment length of the x / y sweeps for a line of n values of S: 32 for
-// complex FP32 lines that split into at least 6 segments of 32 (measured at
-// 96^3 to 256^3), for FP64 real lines of 512 or more (measured at 512^3: the
+// complex FP32 lines that split into at least 7 segments of 32 (measured at
+// 96^3 to 256^3; at 192, 16-element segments are faster since the carry
+// multipliers are tabulated: x 21.7 -> 20.9 us, y 21.8 -> 20.6 us at 192^3),
+// for FP64 real lines of 512 or more (measured at 512^3: the
 // x sweep 532 -> 407 us, the y sweep unchanged) and for lines whose
 // 16-element segments would need more than 512 threads even with 128-byte
 // rows; else 16.  It fixes the rounding of the partitioned solve, so it
@@ -338,7 +340,7 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
 template <typename S>
 static int seg_len(long n) {
     using A = typename Arith<S>::T;
-    if (!std::is_floating_point<A>::value && sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 6) return 32;
+    if (!std::is_floating_point<A>::value && sizeof(A) == 8 && n % 32 == 0 && n / 32 >= 7) return 32;
     if (std::is_same<S, double>::value && n >= 512 && n % 32 == 0) return 32;
     if ((n / 16) * (long)(128 / sizeof(S)) > 512) return 32;
     return 16;
