@@ -172,6 +172,9 @@ struct FusedZ {
     // CU) and 8 % slower in FP64 (its two extra barriers).
     template <typename R>
     static constexpr bool compact() { return std::is_same<R, float>::value; }
+    // row partials of the z integrals: real lines (imaginary time) only
+    template <typename S>
+    static constexpr bool cols() { return std::is_floating_point<S>::value; }
     template <typename S, typename R, int L>
     static constexpr size_t tile_b() { return ((size_t)(N + Pk<S>::n) * L * sizeof(S) + 15) & ~size_t(15); }
     template <typename S, typename R, int L>
@@ -190,18 +193,21 @@ struct FusedZ {
     template <typename S, typename R, int L>
     static constexpr size_t sh_b() {
         using A = typename Arith<S>::T;
-        return (SweepShared<A>::bytes(N, L * (N / 16), L, !exn_in_x<S, R, L>()) + 15) & ~size_t(15);
+        return (SweepShared<A, 16>::bytes(N, L * (N / 16), L, cols<S>() && !exn_in_x<S, R, L>()) + 15) & ~size_t(15);
     }
     // compact layout: tables (or twiddles), last elements, row partials
     template <typename S, typename R, int L>
     static constexpr size_t tab_b() {
-        constexpr size_t t = 4 * (size_t)N * sizeof(typename Arith<S>::T), w = (size_t)N * sizeof(cpx<R>);
+        using A = typename Arith<S>::T;
+        constexpr size_t t = 4 * (size_t)SweepShared<A, 16>::tab_len(N) * sizeof(A), w = (size_t)N * sizeof(cpx<R>);
         return ((t > w ? t : w) + 15) & ~size_t(15);
     }
     template <typename S, typename R, int L>
     static constexpr size_t exl_b() { return ((size_t)L * sizeof(typename Arith<S>::T) + 15) & ~size_t(15); }
     template <typename S, typename R, int L>
-    static constexpr size_t exn_b() { return ((size_t)L * (N / 16) * sizeof(double) + 15) & ~size_t(15); }
+    static constexpr size_t exn_b() {
+        return cols<S>() ? ((size_t)L * (N / 16) * sizeof(double) + 15) & ~size_t(15) : 0;
+    }
     // offset of the tile region
     template <typename S, typename R, int L>
     static constexpr size_t x_off() {
@@ -231,6 +237,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     using Pl = FftPlan<N>;
     static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
     using F = FusedZ<N>;
+    static_assert(COLS == F::template cols<S>(), "z integrals for real lines");
     constexpr bool CMP = F::template compact<R>();  // compact layout, swizzled transform buffer
     constexpr size_t X_OFF = F::template x_off<S, R, L>(), X_B = F::template x_b<S, R, L>();
     static_assert(X_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * (CMP ? LF : LF + 1) * sizeof(cpx<R>),
@@ -239,7 +246,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     const SolveAt sr = solve_at(v);
     S *__restrict__ psi = reinterpret_cast<S *>(sr.psi);
     cpx<R> *__restrict__ spec = reinterpret_cast<cpx<R> *>(sr.spec);
-    SweepShared<A> sh(smem, N, WG, L,
+    SweepShared<A, M> sh(smem, N, WG, L,
                       (!CMP && F::template exn_in_x<S, R, L>()) ? reinterpret_cast<double *>(smem + X_OFF + F::template tile_b<S, R, L>())
                                                                  : nullptr);
     cpx<R> *tw = reinterpret_cast<cpx<R> *>(smem + X_OFF + X_B);

@@ -161,7 +161,7 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
     const int n = AXIS == 0 ? Nx : Ny;
     const int P = n / M;
     const int nt = L * P;
-    SweepShared<A> sh(smem_raw, n, nt, L);
+    SweepShared<A, M> sh(smem_raw, n, nt, L);
     const int tid = threadIdx.x;
     const int l = tid % L, s = tid / L;
     const int zt = (Nz + L - 1) / L;
@@ -219,8 +219,8 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
     const int RS = n + VE;
     const int P = n / M;
     const int nt = L * P;
-    SweepShared<A> sh(smem_raw, n, nt, L);
-    S *tile = reinterpret_cast<S *>(smem_raw + ((SweepShared<A>::bytes(n, nt, L) + 15) & ~size_t(15)));
+    SweepShared<A, M> sh(smem_raw, n, nt, L);
+    S *tile = reinterpret_cast<S *>(smem_raw + ((SweepShared<A, M>::bytes(n, nt, L, COLS) + 15) & ~size_t(15)));
     const int tid = threadIdx.x;
     const int l = tid % L, s = tid / L;
     const int yt = (Ny + L - 1) / L;
@@ -427,7 +427,8 @@ static hipError_t sweep_xy_launch(const Launch &q, int Nx, int Ny, int Nz, const
     if (n % M != 0 || L * P > 512) return hipErrorInvalidValue;
     const long nblocks = (long)((Nz + L - 1) / L) * (AXIS == 0 ? Ny : Nx);
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
-    const size_t lds = SweepShared<A>::bytes(n, L * P, L);
+    const size_t lds =
+        M == 32 ? SweepShared<A, 32>::bytes(n, L * P, L, false) : SweepShared<A, 16>::bytes(n, L * P, L, false);
     if constexpr (sizeof(A) <= 8) {
         if (M == 32) {
             k_sweep_xy<S, AXIS, L, 32, PEEL><<<grid, block, lds, q.s>>>(q.v, Nx, Ny, Nz, t.at(32), k);
@@ -478,7 +479,7 @@ static hipError_t sweep_z_launch(const Launch &q, int Nx, int Ny, int Nz, const 
     const int P = Nz / 16;
     const long nblocks = (long)((Ny + L - 1) / L) * Nx;
     dim3 grid((unsigned)nblocks, (unsigned)q.v.nb), block(L * P);
-    size_t lds = (SweepShared<A>::bytes(Nz, L * P, L) + 15) & ~size_t(15);
+    size_t lds = (SweepShared<A, 16>::bytes(Nz, L * P, L, COLS) + 15) & ~size_t(15);
     if (!DIRECT) lds += (size_t)(Nz + Pk<S>::n) * L * sizeof(S);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_z<S, L, 16, COLS, DIRECT>),

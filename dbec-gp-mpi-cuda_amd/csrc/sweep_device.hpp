@@ -135,25 +135,46 @@ struct SweepTab {
     const typename TabOf<A>::T *ca, *cg, *gb, *fa;
 };
 
-// LDS layout shared by the sweep kernels: the four tables (n each), the
-// carry exchange (one per thread), the last elements, the row partials (one
-// per thread; placed elsewhere when the kernel passes exn_at).
-template <typename A>
+// LDS layout shared by the sweep kernels with segments of M: the four
+// tables, the carry exchange (one per thread), the last elements, the row
+// partials (one per thread, only with the z integrals; placed elsewhere when
+// the kernel passes exn_at).  A table keeps the M entries of segment s in
+// slots s (M + PAD) .. s (M + PAD) + M - 1 (slot()).  The lanes of a
+// wavefront read the same entry of several segments at once: unpadded,
+// segments of 256 bytes (16 complex doubles, 32 complex floats) all start in
+// one bank and segments of 128 bytes in two, and those reads serialise.
+// Complex segments of 128 bytes or more are therefore followed by 16 bytes
+// of padding, which puts the starts of up to 16 consecutive segments in
+// different banks (measured: the fused z kernel at 192^3 complex64 32.9 ->
+// 31.7 us, the z sweep at 256x256x384 complex128 205 -> 199 us).  Real
+// lines take none: FP32 segments of 16 are 64 bytes, and padded FP64 ones
+// made the fused z kernel at 256^3 slower (75 -> 89 us).  With padding, ca
+// holds calpha shifted by one slot (CSH), so a segment reads only its own
+// slots.
+template <typename A, int M>
 struct SweepShared {
+    static constexpr int PAD = std::is_floating_point<A>::value || M * sizeof(A) < 128 ? 0
+                               : sizeof(A) >= 16                                     ? 1
+                                                                                     : 16 / (int)sizeof(A);
+    static constexpr int CSH = PAD > 0 ? 1 : 0;
     A *ca, *cg, *gb, *fa, *exa, *exb, *exl;
     double *exn;
+    __host__ __device__ static constexpr int tab_len(int n) { return n + (n / M) * PAD; }
+    __host__ __device__ static constexpr int slot(int i) { return i + (i / M) * PAD; }
     __device__ SweepShared(char *raw, int n, int nthreads, int L, double *exn_at = nullptr) {
+        const int tl = tab_len(n);
         ca = reinterpret_cast<A *>(raw);
-        cg = ca + n;
-        gb = cg + n;
-        fa = gb + n;
-        exa = fa + n;
+        cg = ca + tl;
+        gb = cg + tl;
+        fa = gb + tl;
+        exa = fa + tl;
         exb = exa + nthreads;
         exl = exb + nthreads;
         exn = exn_at ? exn_at : reinterpret_cast<double *>(exl + L);
     }
-    __host__ __device__ static constexpr size_t bytes(int n, int nthreads, int L, bool with_exn = true) {
-        return (4 * (size_t)n + 2 * (size_t)nthreads + L) * sizeof(A) + (with_exn ? (size_t)nthreads * sizeof(double) : 0);
+    __host__ __device__ static constexpr size_t bytes(int n, int nthreads, int L, bool with_exn) {
+        return (4 * (size_t)tab_len(n) + 2 * (size_t)nthreads + L) * sizeof(A) +
+               (with_exn ? (size_t)nthreads * sizeof(double) : 0);
     }
 };
 
@@ -171,25 +192,28 @@ __device__ __forceinline__ CoefPre<A> coef_pre(const SweepTab<A> &t, int n, int 
     const A z{};
     return {coef_round(z, t.ca[ic]), coef_round(z, t.cg[ic]), coef_round(z, t.gb[i]), coef_round(z, t.fa[i])};
 }
-template <typename A>
-__device__ __forceinline__ void coef_put(const SweepShared<A> &sh, const CoefPre<A> &c, const SweepTab<A> &t, int n,
+template <typename A, int M>
+__device__ __forceinline__ void coef_put(const SweepShared<A, M> &sh, const CoefPre<A> &c, const SweepTab<A> &t, int n,
                                          int tid, int nthreads) {
+    using SH = SweepShared<A, M>;
     if (tid < n) {
+        const int p = SH::slot(tid);
         if (tid < n - 1) {
-            sh.ca[tid] = c.ca;
-            sh.cg[tid] = c.cg;
+            sh.ca[SH::slot(tid + SH::CSH)] = c.ca;
+            sh.cg[p] = c.cg;
         }
-        sh.gb[tid] = c.gb;
-        sh.fa[tid] = c.fa;
+        sh.gb[p] = c.gb;
+        sh.fa[p] = c.fa;
     }
     const A z{};
     for (int i = tid + nthreads; i < n; i += nthreads) {
+        const int p = SH::slot(i);
         if (i < n - 1) {
-            sh.ca[i] = coef_round(z, t.ca[i]);
-            sh.cg[i] = coef_round(z, t.cg[i]);
+            sh.ca[SH::slot(i + SH::CSH)] = coef_round(z, t.ca[i]);
+            sh.cg[p] = coef_round(z, t.cg[i]);
         }
-        sh.gb[i] = coef_round(z, t.gb[i]);
-        sh.fa[i] = coef_round(z, t.fa[i]);
+        sh.gb[p] = coef_round(z, t.gb[i]);
+        sh.fa[p] = coef_round(z, t.fa[i]);
     }
 }
 
@@ -234,18 +258,20 @@ __device__ __forceinline__ cpx<R> sweep_back(const SweepConst<cpx<R>> &k, cpx<R>
 // FMA chain predicated, instead of one LDS round trip per step.
 template <typename A, int M, bool COLS, bool PEEL, int PC = 0>
 __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n, int s, int l, int L, int P,
-                                            const SweepShared<A> &sh, const SweepConst<A> &k,
+                                            const SweepShared<A, M> &sh, const SweepConst<A> &k,
                                             const double *__restrict__ wz) {
     static_assert(M >= 2, "segment length");
     const bool in0 = s > 0, inM = s < P - 1;  // j = 0 / j = M-1 inside 1..n-2
     auto inr = [&](int j) { return PEEL || (e0 + j >= 1 && e0 + j <= n - 2); };  // middle iterations
     const int bot = e0 > 1 ? e0 : 1, top = e0 + M - 1 < n - 2 ? e0 + M - 1 : n - 2;
+    using SH = SweepShared<A, M>;
+    const int tb = SH::PAD ? s * (M + SH::PAD) : e0;  // the segment's first table slot
     const A zero = m_zero(k.A);
     A bq = zero;
     {
         A pn = hr, pc = v[M - 1];
         auto back = [&](int j, A pp) {
-            bq = sweep_back(k, sh.cg[e0 + j], bq, sweep_rhs(k, pn, pc, pp));
+            bq = sweep_back(k, sh.cg[tb + j], bq, sweep_rhs(k, pn, pc, pp));
             v[j] = bq;
         };
         if (inM) back(M - 1, v[M - 2]);
@@ -261,7 +287,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         if (in0) back(0, hl);
     }
     sh.exa[s * L + l] = bq;
-    sh.exb[s * L + l] = sh.gb[bot];  // the segment's product of cgamma A
+    sh.exb[s * L + l] = sh.gb[tb + (bot - e0)];  // the segment's product of cgamma A
     if (s == P - 1) sh.exl[l] = v[M - 1];  // cbeta[N-2] = psi[N-1]
     __syncthreads();
     A x = sh.exl[l];
@@ -279,7 +305,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
     }
     {
-        auto fixb = [&](int j) { v[j] = m_fma(sh.gb[e0 + j], x, v[j]); };
+        auto fixb = [&](int j) { v[j] = m_fma(sh.gb[tb + j], x, v[j]); };
         if (inM) fixb(M - 1);
 #pragma unroll
         for (int j = M - 2; j >= 1; --j)
@@ -290,7 +316,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     A fy = zero;
     {
         auto fwd = [&](int j) {
-            fy = m_fma(sh.ca[e0 + j - 1], fy, v[j]);
+            fy = m_fma(sh.ca[tb + j - 1 + SH::CSH], fy, v[j]);
             v[j] = fy;
         };
         if (in0) fwd(0);
@@ -300,7 +326,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         if (inM) fwd(M - 1);
     }
     sh.exa[s * L + l] = fy;
-    sh.exb[s * L + l] = sh.fa[top];  // the segment's product of calpha
+    sh.exb[s * L + l] = sh.fa[tb + (top - e0)];  // the segment's product of calpha
     __syncthreads();
     A y = zero;  // psi'[0] = 0
     if constexpr (PC > 0) {
@@ -319,7 +345,7 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
     double acc = 0.0;
     {
         auto fixf = [&](int j) {
-            v[j] = m_fma(sh.fa[e0 + j], y, v[j]);
+            v[j] = m_fma(sh.fa[tb + j], y, v[j]);
             if (COLS) acc = fma(wz[e0 + j], m_mod2(v[j]), acc);
         };
         if (in0) fixf(0);
