@@ -189,6 +189,24 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
     }
 }
 
+// Row and column of packet tid + q * nt (VE elements each) of a run of rows
+// of n elements, for q = 0, 1, ...: one division per thread, then a step and
+// a wrap per packet (n is a run-time value: a division per packet costs
+// tens of instructions).
+struct RunPos {
+    int row, e, dr, de;
+    __device__ RunPos(int tid, int nt, int VE, int n)
+        : row(tid * VE / n), e(tid * VE % n), dr(nt * VE / n), de(nt * VE % n) {}
+    __device__ void next(int n) {
+        row += dr;
+        e += de;
+        if (e >= n) {
+            e -= n;
+            ++row;
+        }
+    }
+};
+
 // z sweep: lines are contiguous rows.  A workgroup covers L rows of one x
 // plane.  Staged (DIRECT false): the rows go through an LDS tile in the
 // storage type (row-major, rows padded by one 16-byte packet: coalesced row
@@ -258,13 +276,11 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
             r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
         }
         coef_put(sh, cp, tab, n, tid, nt);
+        RunPos rp(tid, nt, VE, n);
 #pragma unroll
         for (int q = 0; q < KV; ++q) {
-            const int iv = tid + q * nt;
-            if (iv < npk) {
-                const int e = (iv * VE) % n, row = (iv * VE) / n;
-                st_pk(tile + row * RS + e, r[q]);
-            }
+            if (tid + q * nt < npk) st_pk(tile + rp.row * RS + rp.e, r[q]);
+            rp.next(n);
         }
         __syncthreads();
         const S *row = tile + l * RS;
@@ -317,13 +333,12 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
         __syncthreads();
     }
     if constexpr (!DIRECT) {
+        RunPos rp(tid, nt, VE, n);
 #pragma unroll
         for (int q = 0; q < KV; ++q) {
             const int iv = tid + q * nt;
-            if (iv < npk) {
-                const int e = (iv * VE) % n, row = (iv * VE) / n;
-                st_pk(psi + base + (long)iv * VE, ld_pk(tile + row * RS + e));
-            }
+            if (iv < npk) st_pk(psi + base + (long)iv * VE, ld_pk(tile + rp.row * RS + rp.e));
+            rp.next(n);
         }
     }
 }
