@@ -261,6 +261,9 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
                                             const SweepShared<A, M> &sh, const SweepConst<A> &k,
                                             const double *__restrict__ wz) {
     static_assert(M >= 2, "segment length");
+    // carries per group in the run-time folds: 4, or 2 for complex FP32
+    // lines, whose x / y sweeps lose a wave of occupancy with 4
+    constexpr int FB = std::is_same<A, cpx<float>>::value ? 2 : 4;
     const bool in0 = s > 0, inM = s < P - 1;  // j = 0 / j = M-1 inside 1..n-2
     auto inr = [&](int j) { return PEEL || (e0 + j >= 1 && e0 + j <= n - 2); };  // middle iterations
     const int bot = e0 > 1 ? e0 : 1, top = e0 + M - 1 < n - 2 ? e0 + M - 1 : n - 2;
@@ -302,7 +305,20 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         for (int t = PC - 1; t > 0; --t)
             if (t > s) x = m_fma(eb[t], x, ea[t]);
     } else {
-        for (int t = P - 1; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
+        // in groups of FB carries loaded at once (the same FMA order as one
+        // at a time, without an LDS round trip per step)
+        int t = P - 1;
+        for (; t - (FB - 1) > s; t -= FB) {
+            A eb[FB], ea[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                eb[u] = sh.exb[(t - u) * L + l];
+                ea[u] = sh.exa[(t - u) * L + l];
+            }
+#pragma unroll
+            for (int u = 0; u < FB; ++u) x = m_fma(eb[u], x, ea[u]);
+        }
+        for (; t > s; --t) x = m_fma(sh.exb[t * L + l], x, sh.exa[t * L + l]);
     }
     {
         auto fixb = [&](int j) { v[j] = m_fma(sh.gb[tb + j], x, v[j]); };
@@ -340,7 +356,18 @@ __device__ __forceinline__ double seg_solve(A (&v)[M], A hl, A hr, int e0, int n
         for (int t = 0; t < PC - 1; ++t)
             if (t < s) y = m_fma(eb[t], y, ea[t]);
     } else {
-        for (int t = 0; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
+        int t = 0;
+        for (; t + (FB - 1) < s; t += FB) {
+            A eb[FB], ea[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                eb[u] = sh.exb[(t + u) * L + l];
+                ea[u] = sh.exa[(t + u) * L + l];
+            }
+#pragma unroll
+            for (int u = 0; u < FB; ++u) y = m_fma(eb[u], y, ea[u]);
+        }
+        for (; t < s; ++t) y = m_fma(sh.exb[t * L + l], y, sh.exa[t * L + l]);
     }
     double acc = 0.0;
     {
