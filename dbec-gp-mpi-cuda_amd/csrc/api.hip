@@ -660,8 +660,8 @@ int autotune(dbec_ctx *c) {
     CHK(choose(&t.lm[0], {0, 1, 2}, [&] { return launch_zfwd(c->Nz, fft_args(c, 2, v)); }));
     CHK(choose(&t.lm[1], {0, 1, 2}, [&] { return launch_ypass(c->Ny, -1, fft_args(c, 1, v)); }));
     CHK(choose(&t.lm[2], {0, 1, 2}, [&] { return launch_xconv(c->Nx, fft_args(c, 0, v)); }));
-    if (c->mode == DBEC_IMAG && c->prec != DBEC_MIXED) {
-        // the fused inverse pass has two buffer layouts here: its tile
+    if (c->prec != DBEC_MIXED) {
+        // the FP64 fused inverse pass has two buffer layouts: its tile
         // multiplier and layout are chosen together
         const int lms[5] = {0, 1, 2, 1, 2}, sws[5] = {-1, 0, 0, 1, 1};
         std::vector<double> ms;

@@ -634,7 +634,12 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     constexpr int C = PREFETCH ? Q : (Q < 8 ? Q : 8);  // psi batch
     static_assert((2 * L * N) % WG == 0 && Q % C == 0, "zinv_nu tile");
     __shared__ cpx<R> buf[N * lds_slots_per_row<L, SWZ>()];
-    __shared__ cpx<R> tw[N];
+    // complex FP64 psi with the swizzled buffer: the passes read the
+    // twiddles from the global table (L1 / L2), which leaves LDS for a third
+    // workgroup per CU
+    constexpr bool TWG = SWZ && std::is_same<P, cpx<double>>::value;
+    __shared__ cpx<R> tw_s[TWG ? 1 : N];
+    const cpx<R> *tw = TWG ? twg : tw_s;
     __shared__ double sqz[N];      // trap_z of each z
     __shared__ double sqr[2 * L];  // trap_xy of the tile's rows (faces: of each face row)
     const SolveAt sr = solve_at(v);
@@ -649,7 +654,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
     nu_prepare(nc);
     const int tid = threadIdx.x;
     const double s = v.use_scale ? sr.scal[SC_SCALE] : 1.0;
-    stage_twiddles(tw, twg, N, tid, WG);
+    if constexpr (!TWG) stage_twiddles(tw_s, twg, N, tid, WG);
     for (int i = tid; i < N; i += WG) sqz[i] = trap_z(tc, sq[Nx + Ny + i]);
     if ((int)blockIdx.x >= nfaceb) {
         const int b = (int)blockIdx.x - nfaceb;
@@ -775,7 +780,7 @@ inline int fft_lm(const char *name, int dflt) {
     return (v == 1 || v == 2) ? v : dflt;
 }
 // DBEC_FFT_ZINV_SWZ: transform buffer layout of the FP64 fused inverse pass
-// with real psi (0 padded, 1 swizzled); -1 when unset
+// (0 padded, 1 swizzled); -1 when unset
 inline int fft_swz_env() {
     const char *e = getenv("DBEC_FFT_ZINV_SWZ");
     if (!e) return -1;
@@ -960,12 +965,13 @@ struct FftLaunchImpl {
         dim3 grid((unsigned)(nmain + nfaceb), (unsigned)a.v.nb), block(Tl::WG);
         // FP32 spectra load straight into the first pass's registers (faster at
         // every measured size); FP64 ones through LDS (the extra registers
-        // cost more occupancy than the LDS round trip saves).  FP64 with real
-        // psi has both transform buffer layouts (faster at 256, slower at 160
-        // and 192): the environment variable, else the tuned choice, else
-        // padded.
+        // cost more occupancy than the LDS round trip saves).  FP64 has both
+        // transform buffer layouts (swizzled: real psi faster at 256, slower
+        // at 160 and 192; complex psi, twiddles from the global table, 234 ->
+        // 215 us at 256x256x384, slower at 192): the environment variable,
+        // else the tuned choice, else padded.
         constexpr bool ZREG = std::is_same<R, float>::value;
-        if constexpr (std::is_same<R, double>::value && std::is_same<P, double>::value) {
+        if constexpr (std::is_same<R, double>::value) {
             static const int env = fft_swz_env();
             if (env >= 0 ? env == 1 : a.tune.zinv_swz == 1) {
                 k_zinv_nu<N, R, P, LM, ZREG, true><<<grid, block, 0, a.stream>>>(a.v, a.sq, a.nc, (int)a.Nx, (int)a.Ny,

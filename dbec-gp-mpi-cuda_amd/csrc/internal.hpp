@@ -55,7 +55,7 @@ struct Tuning {
     int xy_rowb[2] = {0, 0};    // x / y sweeps: bytes of z per row load (128, 256)
     int xy_peel[2] = {-1, -1};  // x / y sweeps: boundary iterations peeled (1) or tested (0)
     int z_direct = -1;          // z sweep: segments loaded straight from HBM (1) or staged through LDS (0)
-    int zinv_swz = -1;          // fused inverse z pass, FP64 with real psi: swizzled (1) or padded (0) LDS layout
+    int zinv_swz = -1;          // fused inverse z pass, FP64: swizzled (1) or padded (0) LDS layout
 };
 
 struct KernelTimer;

@@ -65,23 +65,25 @@ def test_tuned_choices_leave_results_bit_identical():
     assert run({}) == rules
 
 
-# The FP64 fused inverse z pass with real psi has two LDS layouts (padded and
-# XOR-swizzled); each z length has its own radix plan, and the plans' first
-# radices (4, 8, 16) set the swizzle.
+# The FP64 fused inverse z pass has two LDS layouts (padded and XOR-swizzled;
+# with complex psi the swizzled one also reads the twiddles from the global
+# table); each z length has its own radix plan, and the plans' first radices
+# (4, 8, 16) set the swizzle.
 LAYOUT_SCRIPT = r"""
 import hashlib, json, sys
 sys.path.insert(0, sys.argv[1])
 import dbecpy
 lib = dbecpy.load(dbecpy.HIP_LIB)
 out = {}
-for nz in (48, 80, 96, 192, 256, 384, 512):
-    p = dbecpy.make_params(32, 48, nz, 0.25, 0.25, 0.1, 0.002, 0.5, 1.0, 1.5, 6.0)
-    with dbecpy.Solver(lib, p, dbecpy.IMAG, dbecpy.FP64) as s:
-        s.set_psi(dbecpy.gaussian_psi0(p))
-        s.calcnorm(rescale=True)
-        s.set_coupling(2 * 20.0, 2 * 10.0)
-        s.step(4)
-        out[str(nz)] = hashlib.sha256(s.get_psi().tobytes()).hexdigest()
+for mode, name in ((dbecpy.IMAG, "imag"), (dbecpy.REAL, "real")):
+    for nz in (48, 80, 96, 192, 256, 384, 512):
+        p = dbecpy.make_params(32, 48, nz, 0.25, 0.25, 0.1, 0.002, 0.5, 1.0, 1.5, 6.0)
+        with dbecpy.Solver(lib, p, mode, dbecpy.FP64) as s:
+            s.set_psi(dbecpy.gaussian_psi0(p))
+            s.calcnorm(rescale=True)
+            s.set_coupling(2 * 20.0, 2 * 10.0)
+            s.step(4)
+            out[name + str(nz)] = hashlib.sha256(s.get_psi().tobytes()).hexdigest()
 print(json.dumps(out))
 """
 
@@ -97,5 +99,5 @@ def test_fused_inverse_layouts_bit_identical():
         assert r.returncode == 0, r.stderr[-2000:]
         return json.loads(r.stdout.strip().splitlines()[-1])
     padded = layout("0")
-    assert len(padded) == 7
+    assert len(padded) == 14
     assert layout("1") == padded
