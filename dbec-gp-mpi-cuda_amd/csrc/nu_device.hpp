@@ -18,10 +18,9 @@ struct TrapConst {
 };
 struct NuConst {
     double dt, g, gd, nd;  // nd = Nx*Ny*Nz
-    double inv_nd;         // 1/nd, used when nd is a power of two (then phi*inv_nd == phi/nd exactly)
+    double nd_odd, inv_odd, pow2_inv;  // nd = nd_odd 2^e with nd_odd odd; RN(1 / nd_odd); 2^-e (nd_split)
     double shift;          // imaginary time: exponent shift (0 = the reference's factor)
     double *capped;        // set to 1 where the exponent was capped
-    int nd_pow2;
     int cap;               // cap the exponent (mixed form)
     float dtf, gf, gdnf, shiftf;  // mixed form: dt, g, gd / nd, shift in FP32 (nu_prepare)
 };
@@ -34,8 +33,30 @@ __host__ __device__ inline void nu_prepare(NuConst &nc) {
     nc.shiftf = (float)nc.shift;
 }
 
+// Splits nd = nd_odd 2^e for div_nd.
+__host__ __device__ inline void nd_split(NuConst &nc, long nd) {
+    double p = 1.0;
+    while (nd % 2 == 0) {
+        nd /= 2;
+        p *= 0.5;
+    }
+    nc.nd_odd = (double)nd;
+    nc.inv_odd = 1.0 / (double)nd;
+    nc.pow2_inv = p;
+}
+
+// phi / nd rounded to nearest, equal to the IEEE quotient, without a divide
+// instruction sequence.  With b = nd_odd < 2^50 and y = RN(1/b):
+// q = RN(phi y) is within 1.5 ulp u of phi / b; the remainder r = phi - b q is
+// a multiple of u below 2^51 u, so the FMA forms it exactly; q + r y differs
+// from phi / b by |r| |y - 1/b| <= 1.5 2^-53 u, while phi / b lies at least
+// u / (2 b) from every midpoint between doubles (2 phi is an even multiple
+// of u and b k u an odd one for odd k), so RN(q + r y) = RN(phi / b).  The
+// factor 2^-e is exact in the normal range.  For b = 1 this is phi 2^-e.
 __device__ __forceinline__ double div_nd(double phi, const NuConst &nc) {
-    return nc.nd_pow2 ? phi * nc.inv_nd : phi / nc.nd;
+    const double q = phi * nc.inv_odd;
+    const double r = fma(-q, nc.nd_odd, phi);
+    return fma(r, nc.inv_odd, q) * nc.pow2_inv;
 }
 
 // 0.5*par*(g^2 x^2 + n^2 y^2 + l^2 z^2) in the reference's evaluation order

@@ -80,8 +80,7 @@ NuConst nu_const(const dbec_ctx *c) {
     nc.gd = c->gd;
     const long nd = c->Nx * c->Ny * c->Nz;
     nc.nd = (double)nd;
-    nc.inv_nd = 1.0 / (double)nd;
-    nc.nd_pow2 = (nd & (nd - 1)) == 0;
+    nd_split(nc, nd);
     nc.shift = 0.0;
     nc.cap = 0;
     nc.capped = nullptr;
