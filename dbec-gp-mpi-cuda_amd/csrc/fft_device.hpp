@@ -194,8 +194,9 @@ constexpr int ilog2(int v) { return v > 1 ? 1 + ilog2(v / 2) : 0; }
 // copies) over all banks, and the other passes touch four consecutive rows of
 // every line at once; on 64 banks of 4 bytes, serving 256 bytes per cycle,
 // that removes the bank conflicts the padded layout has in the passes.  It
-// costs address registers: measured, it pays only where the conflicts
-// dominate (FP64 zinv_nu with real psi).
+// costs address registers: measured, it pays only in the FP64 fused inverse
+// pass (real psi at 256; complex psi at 384, where dropping the padding and
+// the LDS twiddles lets a third workgroup onto each CU).
 template <int L, bool SWZ>
 constexpr int lds_slots_per_row() { return SWZ ? L : L + 1; }
 
