@@ -172,19 +172,25 @@ k_sweep_xy(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab
     const int e0 = s * M;
     const A zero = m_zero(k.A);
     const CoefPre<A> cp = coef_pre(tab, n, tid);
+    // The segment's elements at 32-bit byte offsets j * es from its first one
+    // (at most 16 * 512^2 * 16 bytes): one 64-bit address per thread, then a
+    // 32-bit step per element instead of a 64-bit multiply-add.
+    char *const seg = reinterpret_cast<char *>(psi + base + (long)e0 * es);
+    const unsigned bes = (unsigned)(es * (long)sizeof(S));
+    auto at = [&](int j) -> S & { return *reinterpret_cast<S *>(seg + (unsigned)j * bes); };
     // Predicated loads converted in place: fewer registers (higher occupancy)
     // than issuing them into a separate array, which measured faster here.
     A v[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? to_a(psi[base + (long)(e0 + j) * es]) : zero;
-    const A hl = (act && e0 > 0) ? to_a(psi[base + (long)(e0 - 1) * es]) : zero;
-    const A hr = (act && e0 + M < n) ? to_a(psi[base + (long)(e0 + M) * es]) : zero;
+    for (int j = 0; j < M; ++j) v[j] = act ? to_a(at(j)) : zero;
+    const A hl = (act && e0 > 0) ? to_a(*reinterpret_cast<const S *>(seg - bes)) : zero;
+    const A hr = (act && e0 + M < n) ? to_a(at(M)) : zero;
     coef_put(sh, cp, tab, n, tid, nt);
     __syncthreads();
     seg_solve<A, M, false, PEEL>(v, hl, hr, e0, n, s, l, L, P, sh, k, nullptr);
     if (act) {
 #pragma unroll
-        for (int j = 0; j < M; ++j) psi[base + (long)(e0 + j) * es] = from_a<S>(v[j]);
+        for (int j = 0; j < M; ++j) at(j) = from_a<S>(v[j]);
     }
 }
 
