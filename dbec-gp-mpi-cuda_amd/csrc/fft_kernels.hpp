@@ -105,8 +105,8 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
     for (int k = 0; k < KV; ++k) {
         const int iv = tid + k * WG;
         const int p = iv / NVR, zv = iv % NVR;
-        ra[k] = *reinterpret_cast<const Pv *>(src + (long)min(2 * p, nrows - 1) * N + zv * VE);
-        rb[k] = *reinterpret_cast<const Pv *>(src + (long)min(2 * p + 1, nrows - 1) * N + zv * VE);
+        ra[k] = *reinterpret_cast<const Pv *>(src + (unsigned)(min(2 * p, nrows - 1) * N + zv * VE));
+        rb[k] = *reinterpret_cast<const Pv *>(src + (unsigned)(min(2 * p + 1, nrows - 1) * N + zv * VE));
     }
 #pragma unroll
     for (int k = 0; k < KV; ++k) {
@@ -135,7 +135,7 @@ k_zfwd(Solves v, int Nx, int Ny, const cpx<R> *__restrict__ twg, const double *_
             cpx<R> o;
             if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
             else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
-            spec[((long)k * Nx + x) * Ny + y0 + row] = o;
+            spec[(unsigned)((k * Nx + x) * Ny + y0 + row)] = o;
         }
     }
     if (v.fold == 1 && blockIdx.x == 0) {  // the last step's norm, for the x convolution and calcnu
@@ -277,7 +277,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
 #pragma unroll
         for (int q = 0; q < KV; ++q) {
             const int iv = tid + q * WG;
-            r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
+            r[q] = ld_pk(psi + base + (unsigned)((iv < npk ? iv : 0) * VE));
         }
         if constexpr (CMP) {
 #pragma unroll
@@ -350,7 +350,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
         const int iv = tid + q * WG;
         if (iv < npk) {
             const int e = (iv * VE) % N, row = (iv * VE) / N;
-            st_pk(psi + base + (long)iv * VE, ld_pk(tile + row * RS + e));
+            st_pk(psi + base + (unsigned)(iv * VE), ld_pk(tile + row * RS + e));
         }
     }
     __syncthreads();  // the tile is stored; its LDS becomes the transform buffer (and the tables the twiddles)
@@ -384,7 +384,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
             cpx<R> o;
             if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
             else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
-            spec[((long)kz * Nx + xrow) * Ny + y0 + row] = o;
+            spec[(unsigned)((kz * Nx + xrow) * Ny + y0 + row)] = o;
         }
     }
 }
@@ -468,7 +468,7 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     cpx<R> *g = spec + (long)kz * N * Ny + y;
     cpx<R> r[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) r[k] = g[(long)(t + k * T) * Ny];
+    for (int k = 0; k < K; ++k) r[k] = g[(unsigned)((t + k * T) * Ny)];
     // potdd(|kx|, |ky|, kz): the table rows [kz][|ky|] are contiguous in |kx|
     const int Ny2 = Ny / 2, Nyh = Ny2 + 1;
     const R *pk = potddq + ((long)kz * Nyh + (y <= Ny2 ? y : Ny - y)) * NXH;
@@ -488,7 +488,7 @@ k_xconv(Solves v, const R *__restrict__ potddq, int Ny, const cpx<R> *__restrict
     FftPlan<N>::template run_reg<+1, L>(buf, l, t, tw, r);
     if (l < nl) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) g[(long)(t + k * T) * Ny] = r[k];
+        for (int k = 0; k < K; ++k) g[(unsigned)((t + k * T) * Ny)] = r[k];
     }
 }
 
@@ -511,10 +511,10 @@ __device__ __forceinline__ void load_hermitian_pairs(cpx<R> *buf, const cpx<R> *
     for (int q = 0; q < K; ++q) {
         const int i = tid + q * WG;
         const int p = i % L, k = i / L;
-        long ia = 0, ib = 0;
+        unsigned ia = 0, ib = 0;
         m[q] = i < nkz * L ? idx(p, k, ia, ib) : 0;
-        ra[q] = spec[(m[q] & 1) ? ia : 0];
-        rb[q] = spec[(m[q] & 2) ? ib : 0];
+        ra[q] = spec[(m[q] & 1) ? ia : 0u];
+        rb[q] = spec[(m[q] & 2) ? ib : 0u];
     }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -546,10 +546,10 @@ __device__ __forceinline__ void load_hermitian_reg(cpx<R> *v, const cpx<R> *__re
 #pragma unroll
     for (int r = 0; r < K; ++r) {
         const int k = t + r * T, kk = k <= N / 2 ? k : N - k;
-        long ia = 0, ib = 0;
+        unsigned ia = 0, ib = 0;
         m[r] = idx(l, kk, ia, ib);
-        ra[r] = spec[(m[r] & 1) ? ia : 0];
-        rb[r] = spec[(m[r] & 2) ? ib : 0];
+        ra[r] = spec[(m[r] & 1) ? ia : 0u];
+        rb[r] = spec[(m[r] & 2) ? ib : 0u];
     }
 #pragma unroll
     for (int r = 0; r < K; ++r) {
@@ -578,8 +578,8 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
     const int y0 = (blockIdx.x % ytiles) * 2 * L;
     const int nrows = min(2 * L, Ny - y0);
     stage_twiddles(tw, twg, N, tid, WG);
-    load_hermitian_pairs<N, L, WG, false>(buf, spec, tid, [&](int p, int k, long &ia, long &ib) {
-        ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+    load_hermitian_pairs<N, L, WG, false>(buf, spec, tid, [&](int p, int k, unsigned &ia, unsigned &ib) {
+        ia = (unsigned)((k * Nx + x) * Ny + y0 + 2 * p);
         ib = ia + 1;
         return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
     });
@@ -593,7 +593,7 @@ k_zinv(const cpx<R> *__restrict__ spec, R *__restrict__ phi, int Nx, int Ny, con
         const int z = i % N, row = i / N;
         if (row < nrows) {
             const cpx<R> c = buf[Pl::template at<L, R>(z, row >> 1)];
-            dst[(long)row * N + z] = (row & 1) ? c.y : c.x;
+            dst[(unsigned)(row * N + z)] = (row & 1) ? c.y : c.x;
         }
     }
 }
@@ -671,11 +671,11 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int i = tid + q * WG;
-                pv[q] = dst[(long)min(i / N, nrows - 1) * N + i % N];
+                pv[q] = dst[(unsigned)(min(i / N, nrows - 1) * N + i % N)];
             }
         }
-        const auto idx = [&](int p, int k, long &ia, long &ib) {
-            ia = ((long)k * Nx + x) * Ny + y0 + 2 * p;
+        const auto idx = [&](int p, int k, unsigned &ia, unsigned &ib) {
+            ia = (unsigned)((k * Nx + x) * Ny + y0 + 2 * p);
             ib = ia + 1;
             return (2 * p < nrows ? 1 : 0) | (2 * p + 1 < nrows ? 2 : 0);
         };
@@ -694,7 +694,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
                     const int i = tid + (q0 + c) * WG;
-                    pv[c] = dst[(long)min(i / N, nrows - 1) * N + i % N];
+                    pv[c] = dst[(unsigned)(min(i / N, nrows - 1) * N + i % N)];
                 }
             }
 #pragma unroll
@@ -705,7 +705,7 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
                     const cpx<R> cv = buf[Pl::template at<L, R, SWZ>(z == N - 1 ? 0 : z, row >> 1)];
                     const R phi = (row & 1) ? cv.y : cv.x;
                     const double pot = trap_sum(tc, sqr[row], sqz[z]);
-                    dst[(long)row * N + z] = nu_apply(pv[PREFETCH ? q0 + c : c], s, phi, pot, nc);
+                    dst[(unsigned)(row * N + z)] = nu_apply(pv[PREFETCH ? q0 + c : c], s, phi, pot, nc);
                 }
             }
         }
@@ -719,14 +719,14 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
         const int f = f0 + min(tid, nl - 1);
         sqr[tid] = trap_xy(tc, sq[f < Ny ? Nx - 1 : f - Ny], sq[Nx + (f < Ny ? f : Ny - 1)]);
     }
-    const auto fidx = [&](int p, int k, long &ia, long &ib) {
+    const auto fidx = [&](int p, int k, unsigned &ia, unsigned &ib) {
         if (p >= nl) return 0;
         const int f = f0 + p;
         int xo, yo, xs, ys;
         if (f < Ny) { xo = Nx - 1; yo = f; xs = 0; ys = f; }
         else { xo = f - Ny; yo = Ny - 1; xs = f - Ny; ys = 0; }
-        ia = ((long)k * Nx + xo) * Ny + yo;
-        ib = ((long)k * Nx + xs) * Ny + ys;
+        ia = (unsigned)((k * Nx + xo) * Ny + yo);
+        ib = (unsigned)((k * Nx + xs) * Ny + ys);
         return 3;
     };
     if constexpr (ZREG) {
@@ -744,13 +744,13 @@ k_zinv_nu(Solves v, const double *__restrict__ sq, NuConst nc, int Nx, int Ny, i
 #pragma unroll
     for (int q0 = 0; q0 < QF; q0 += CF) {
         P fv[CF];
-        long fn[CF];
+        unsigned fn[CF];
 #pragma unroll
         for (int c = 0; c < CF; ++c) {
             const int i = tid + (q0 + c) * WG;
             const int z = i % N, f = f0 + min(i / N, nl - 1);
             const int xo = f < Ny ? Nx - 1 : f - Ny, yo = f < Ny ? f : Ny - 1;
-            fn[c] = ((long)xo * Ny + yo) * N + z;
+            fn[c] = (unsigned)((xo * Ny + yo) * N + z);
             fv[c] = psi[fn[c]];
         }
 #pragma unroll

@@ -278,7 +278,7 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
 #pragma unroll
         for (int q = 0; q < KV; ++q) {
             const int iv = tid + q * nt;
-            r[q] = ld_pk(psi + base + (long)(iv < npk ? iv : 0) * VE);
+            r[q] = ld_pk(psi + base + (unsigned)((iv < npk ? iv : 0) * VE));
         }
         coef_put(sh, cp, tab, n, tid, nt);
         RunPos rp(tid, nt, VE, n);
@@ -342,7 +342,7 @@ k_sweep_z(Solves sv, int Nx, int Ny, int Nz, SweepTab<typename Arith<S>::T> tab,
 #pragma unroll
         for (int q = 0; q < KV; ++q) {
             const int iv = tid + q * nt;
-            if (iv < npk) st_pk(psi + base + (long)iv * VE, ld_pk(tile + rp.row * RS + rp.e));
+            if (iv < npk) st_pk(psi + base + (unsigned)(iv * VE), ld_pk(tile + rp.row * RS + rp.e));
             rp.next(n);
         }
     }
