@@ -175,10 +175,12 @@ def serve_bench(lib, count, baseline_count, rate, max_batch, check_every, timeou
         fp64 = [ds.make_request(r.id, r.nx, r.mode, dbecpy.FP64, r.g / r.par, r.gd / r.par,
                                 (r.vgamma, r.vnu, r.vlambda), r.dt, r.max_steps, r.mutol) for _, r in sub]
         service = []
-        for q in fp64:
+        for i, q in enumerate(fp64):
             t1 = time.perf_counter()
             ds.plain_solve(lib, q, check_every=check_every)
             service.append(time.perf_counter() - t1)
+            if (i + 1) % 200 == 0 and i + 1 < len(fp64):
+                log(f"back-to-back FP64: {i + 1} of {len(fp64)} requests, {sum(service):.1f} s")
         # FIFO single server: start = max(arrival, previous finish)
         fin, blat = 0.0, []
         for (arr, _), sv in zip(sub, service):
