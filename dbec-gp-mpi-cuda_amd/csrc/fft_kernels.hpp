@@ -177,10 +177,17 @@ struct FusedZ {
     static constexpr bool cols() { return std::is_floating_point<S>::value; }
     template <typename S, typename R, int L>
     static constexpr size_t tile_b() { return ((size_t)(N + Pk<S>::n) * L * sizeof(S) + 15) & ~size_t(15); }
+    // transform buffer swizzled (unpadded) only in the compact layout and
+    // only where the padded one would not fit in the tile's region (FP32
+    // real lines); complex FP32 rows are wide enough for the padded buffer
+    template <typename S, typename R, int L>
+    static constexpr bool swz() {
+        return compact<R>() && (size_t)N * (L / 2 + 1) * sizeof(cpx<R>) > tile_b<S, R, L>();
+    }
     template <typename S, typename R, int L>
     static constexpr size_t x_b() {
         constexpr size_t tile = tile_b<S, R, L>();
-        constexpr size_t buf = (size_t)N * (compact<R>() ? L / 2 : L / 2 + 1) * sizeof(cpx<R>);
+        constexpr size_t buf = (size_t)N * (swz<S, R, L>() ? L / 2 : L / 2 + 1) * sizeof(cpx<R>);
         constexpr size_t ex = compact<R>() ? 2 * (size_t)L * (N / 16) * sizeof(typename Arith<S>::T) : 0;
         constexpr size_t m = tile > buf ? tile : buf;
         return ((m > ex ? m : ex) + 15) & ~size_t(15);
@@ -240,7 +247,8 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     static_assert(COLS == F::template cols<S>(), "z integrals for real lines");
     constexpr bool CMP = F::template compact<R>();  // compact layout, swizzled transform buffer
     constexpr size_t X_OFF = F::template x_off<S, R, L>(), X_B = F::template x_b<S, R, L>();
-    static_assert(X_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * (CMP ? LF : LF + 1) * sizeof(cpx<R>),
+    constexpr bool SWZ = F::template swz<S, R, L>();  // swizzled transform buffer
+    static_assert(X_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * (SWZ ? LF : LF + 1) * sizeof(cpx<R>),
                   "fused z LDS");
     __shared__ __align__(16) char smem[F::template lds<S, R, L>()];
     const SolveAt sr = solve_at(v);
@@ -366,11 +374,11 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         const R m2 = act ? (R)PsiTraits<S>::mod2(from_a<S>(vv[j]), 1.0) : R(0);
-        R *slot = reinterpret_cast<R *>(&buf[Pl::template at<LF, R, CMP>(e0 + j, l >> 1)]);
+        R *slot = reinterpret_cast<R *>(&buf[Pl::template at<LF, R, SWZ>(e0 + j, l >> 1)]);
         slot[l & 1] = m2;
     }
     __syncthreads();
-    Pl::template run<-1, LF, CMP>(buf, tid % LF, tid / LF, tw);
+    Pl::template run<-1, LF, SWZ>(buf, tid % LF, tid / LF, tw);
     // split Z = A + iB into the spectra of the two real rows (as k_zfwd)
     constexpr int nkz = N / 2 + 1;
 #pragma unroll
@@ -379,8 +387,8 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
         const int row = i % L, kz = i / L;
         if (kz < nkz && row < nl) {
             const int p = row >> 1;
-            const cpx<R> z = buf[Pl::template at<LF, R, CMP>(kz, p)];
-            const cpx<R> w = buf[Pl::template at<LF, R, CMP>((N - kz) % N, p)];
+            const cpx<R> z = buf[Pl::template at<LF, R, SWZ>(kz, p)];
+            const cpx<R> w = buf[Pl::template at<LF, R, SWZ>((N - kz) % N, p)];
             cpx<R> o;
             if ((row & 1) == 0) o = {R(0.5) * (z.x + w.x), R(0.5) * (z.y - w.y)};
             else o = {R(0.5) * (z.y + w.y), R(0.5) * (w.x - z.x)};
