@@ -166,8 +166,10 @@ struct FusedZ {
     // exchange.  FP32 spectra (compact: R = float): the tables, which the
     // twiddles overwrite once the solve is done; the rows' last elements;
     // the row partials; the tile region, which holds the carry exchange
-    // while the tile sits in registers and the transform buffer (swizzled,
-    // unpadded) after the tile is stored.  Measured at 256^3 and 192^3: the
+    // while the tile sits in registers and the transform buffer after the
+    // tile is stored (swizzled and unpadded where the padded one would not
+    // fit the tile's region: FP32 real lines; padded for complex FP32 rows,
+    // which are wide enough).  Measured at 256^3 and 192^3: the
     // compact layout is 4-5 % faster with FP32 spectra (more workgroups per
     // CU) and 8 % slower in FP64 (its two extra barriers).
     template <typename R>
@@ -245,7 +247,7 @@ k_sweepz_zfwd(Solves v, int Nx, int Ny, SweepTab<typename Arith<S>::T> tab, Swee
     static_assert(FftPlan<N>::T * LF == WG, "one transform thread per sweep thread");
     using F = FusedZ<N>;
     static_assert(COLS == F::template cols<S>(), "z integrals for real lines");
-    constexpr bool CMP = F::template compact<R>();  // compact layout, swizzled transform buffer
+    constexpr bool CMP = F::template compact<R>();  // compact layout
     constexpr size_t X_OFF = F::template x_off<S, R, L>(), X_B = F::template x_b<S, R, L>();
     constexpr bool SWZ = F::template swz<S, R, L>();  // swizzled transform buffer
     static_assert(X_B >= (size_t)RS * L * sizeof(S) && X_B >= (size_t)N * (SWZ ? LF : LF + 1) * sizeof(cpx<R>),
